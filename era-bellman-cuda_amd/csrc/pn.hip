@@ -1,0 +1,121 @@
+// PLONK permutation polynomials for gfx950 (src/pn.cu:17-97, src/pn_kernels.cu:21-121).
+//
+// Cells are visited row-major (p = row*cols + col); a cell's key is its
+// variable index (column-major input).  A stable radix sort of (key, cell)
+// on the low ceil(log2 cols) + log_rows key bits groups the cells of each
+// variable, row-major inside the group.  Each group is a cycle: every cell
+// receives the identity value omega_rows^row * k_col of the NEXT cell of its
+// group (cyclically), except the first group (smallest key), which keeps the
+// identity.  The output is column-major, canonical Montgomery Fr.
+#include "common.h"
+#include "scan.h"
+#include "sort.h"
+#include "tables.h"
+
+namespace bc {
+
+namespace {
+
+constexpr unsigned PN_BLOCK = 256;
+
+// keys[row*cols + col] = indexes[col*rows + row]; cells[...] = col*rows + row
+__global__ void pn_transpose_kernel(const uint32_t *indexes, uint32_t *keys, uint32_t *cells, unsigned cols, unsigned log_rows) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned cells_count = cols << log_rows;
+  if (p >= cells_count)
+    return;
+  const unsigned row = p / cols, col = p % cols;
+  const unsigned cell = (col << log_rows) + row;
+  keys[p] = indexes[cell];
+  cells[p] = cell;
+}
+
+__global__ void pn_run_flags_kernel(const uint32_t *sorted_keys, uint32_t *flags, unsigned n) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n)
+    flags[j] = (j == 0 || sorted_keys[j] != sorted_keys[j - 1]) ? 1u : 0u;
+}
+
+// run_of[j] = index of the run holding sorted position j; start/end of each run
+__global__ void pn_runs_kernel(const uint32_t *flags, const uint32_t *excl, uint32_t *run_of, uint32_t *run_start, uint32_t *run_end, unsigned n) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n)
+    return;
+  const uint32_t r = excl[j] + flags[j] - 1;
+  run_of[j] = r;
+  if (flags[j])
+    run_start[r] = j;
+  if (j == n - 1 || flags[j + 1])
+    run_end[r] = j + 1;
+}
+
+__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint32_t *sorted_cells, const uint32_t *run_of, const uint32_t *run_start,
+                                   const uint32_t *run_end, fe *target, unsigned n, unsigned log_rows) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n)
+    return;
+  const uint32_t r = run_of[j], o = run_start[r], len = run_end[r] - o;
+  const uint32_t src_in = j - o;
+  const uint32_t dst_in = r == 0 ? src_in : (src_in + len - 1) % len;
+  const uint32_t src_cell = sorted_cells[j];
+  const uint32_t dst_cell = sorted_cells[o + dst_in];
+  const uint32_t row = src_cell & ((1u << log_rows) - 1), col = src_cell >> log_rows;
+  const fe tw = table_pow(w, (row << (POW_LOG_RANGE - log_rows)) & ((1u << POW_LOG_RANGE) - 1));
+  store_fe(target + dst_cell, mul<FrParams>(tw, load_fe(scalars + col)));
+}
+
+} // namespace
+
+hipError_t pn_generate(const generate_permutation_polynomials_configuration &cfg) {
+  const ff_state *ff = ff_current();
+  if (!ff)
+    return hipErrorNotInitialized;
+  const unsigned cols = cfg.columns_count, L = cfg.log_rows_count;
+  if ((cols != 3 && cols != 4) || L > 28)
+    return hipErrorInvalidValue;
+  hipStream_t s = to_stream(cfg.stream);
+  hipMemPool_t pool = to_pool(cfg.mem_pool);
+  const unsigned n = cols << L;
+  const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
+  uint32_t *keys = nullptr, *cells = nullptr, *skeys = nullptr, *scells = nullptr, *flags = nullptr, *excl = nullptr;
+  uint32_t *run_of = nullptr, *run_start = nullptr, *run_end = nullptr;
+  BC_TRY(pool_alloc(&keys, n, pool, s));
+  BC_TRY(pool_alloc(&cells, n, pool, s));
+  BC_TRY(pool_alloc(&skeys, n, pool, s));
+  BC_TRY(pool_alloc(&scells, n, pool, s));
+  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(cfg.indexes, keys, cells, cols, L);
+  BC_LAUNCH_CHECK();
+  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, 0, bits, pool, s));
+  BC_TRY(pool_free(keys, s));
+  BC_TRY(pool_free(cells, s));
+  BC_TRY(pool_alloc(&flags, n, pool, s));
+  BC_TRY(pool_alloc(&excl, n, pool, s));
+  pn_run_flags_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(skeys, flags, n);
+  BC_LAUNCH_CHECK();
+  BC_TRY(scan_u32_exclusive(flags, excl, n, pool, s));
+  BC_TRY(pool_alloc(&run_of, n, pool, s));
+  BC_TRY(pool_alloc(&run_start, n, pool, s));
+  BC_TRY(pool_alloc(&run_end, n, pool, s));
+  pn_runs_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(flags, excl, run_of, run_start, run_end, n);
+  BC_LAUNCH_CHECK();
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, static_cast<const fe *>(cfg.scalars), scells, run_of, run_start, run_end,
+                                                                static_cast<fe *>(cfg.target), n, L);
+  BC_LAUNCH_CHECK();
+  for (uint32_t *p : {skeys, scells, flags, excl, run_of, run_start, run_end})
+    BC_TRY(pool_free(p, s));
+  return hipSuccess;
+}
+
+} // namespace bc
+
+using namespace bc;
+
+extern "C" {
+
+bc_error pn_set_up() { return ff_current() ? bc_success : bc_error_invalid_value; }
+
+bc_error pn_generate_permutation_polynomials(generate_permutation_polynomials_configuration configuration) { return as_bc(pn_generate(configuration)); }
+
+bc_error pn_tear_down() { return bc_success; }
+
+} // extern "C"

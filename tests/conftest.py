@@ -24,3 +24,19 @@ def oracle():
         subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
     O.lib()
     return O
+
+
+@pytest.fixture(scope="session")
+def bellman():
+    """The HIP library with ff/ntt/pn/msm set up on device 0 (GPU tests only)."""
+    import ctypes
+
+    import bellman_amd as B
+
+    B.call("ff_set_up", ctypes.c_uint(25), ctypes.c_uint(14))
+    B.call("ntt_set_up")
+    B.call("pn_set_up")
+    B.call("msm_set_up")
+    dev = B.Device(0)
+    yield B, dev
+    dev.close()

@@ -1,0 +1,93 @@
+"""MSM parity through the C ABI.  The 254 per-bit results are compared with
+the CPU oracle projectively (after affine normalisation: the reference's raw
+Jacobian limbs are not canonical), and their fold against the closed form
+[sum k_i s_i]G for bases P_i = [s_i]G (tests/msm_test.cu:107-141)."""
+import numpy as np
+import pytest
+
+from gpu_helpers import msm
+
+pytestmark = pytest.mark.gpu
+
+
+def closed_form(oracle, s_ints, k_mont):
+    ks = oracle.from_mont(k_mont)
+    return oracle.mul_generator(sum(a * b for a, b in zip(ks, s_ints)) % oracle.R)
+
+
+@pytest.mark.parametrize("log_n", [0, 1, 4, 7, 10])
+def test_msm_per_bit_results_match_oracle(bellman, oracle, log_n):
+    B, dev = bellman
+    n = 1 << log_n
+    s = oracle.from_limbs(oracle.random_fr(0xBC000020 + log_n, n, montgomery=False))
+    bases = oracle.gen_bases(s)
+    k = oracle.msm_scalars(0xBC000021 + log_n, n)
+    db = dev.upload(bases)
+    got = msm(dev, db, k, log_n)
+    dev.free(db)
+    exp = oracle.msm(bases, k, log_n)
+    assert np.array_equal(oracle.to_affine(got), oracle.to_affine(exp))
+    assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, s, k))
+
+
+@pytest.mark.parametrize("log_n", [12, 14, 16])
+def test_msm_fold_closed_form(bellman, oracle, log_n):
+    B, dev = bellman
+    n = 1 << log_n
+    s0, d = 0x1234567 + log_n, 0xABCDEF1
+    bases = oracle.gen_bases_progression(s0, d, n)
+    k = oracle.msm_scalars(0xBC000022 + log_n, n)
+    db = dev.upload(bases)
+    got = msm(dev, db, k, log_n)
+    dev.free(db)
+    s = [(s0 + i * d) % oracle.R for i in range(n)]
+    assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, s, k))
+
+
+@pytest.mark.parametrize("host_scalars,host_results", [(True, False), (False, True), (True, True)])
+def test_msm_host_pointers(bellman, oracle, host_scalars, host_results):
+    B, dev = bellman
+    L = 9
+    n = 1 << L
+    s = oracle.from_limbs(oracle.random_fr(31, n, montgomery=False))
+    bases = oracle.gen_bases(s)
+    k = oracle.msm_scalars(32, n)
+    db = dev.upload(bases)
+    got = msm(dev, db, k, L, host_scalars=host_scalars, host_results=host_results)
+    dev.free(db)
+    assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, s, k))
+
+
+def test_msm_edge_scalars(bellman, oracle):
+    """All-zero, all-one, r-1 and repeated bases (doubling inside a bucket)."""
+    B, dev = bellman
+    L = 6
+    n = 1 << L
+    R = oracle.R
+    s = [5] * (n // 2) + [(7 + i) for i in range(n // 2)]  # repeated base P = [5]G
+    bases = oracle.gen_bases(s)
+    db = dev.upload(bases)
+    for ks in ([0] * n, [1] * n, [R - 1] * n, [(i * 0x9E3779B97F4A7C15) % R for i in range(n)]):
+        k = oracle.to_mont(ks)
+        got = msm(dev, db, k, L)
+        exp = closed_form(oracle, s, k)
+        assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], exp)
+        assert np.array_equal(oracle.to_affine(got), oracle.to_affine(oracle.msm(bases, k, L)))
+    dev.free(db)
+
+
+def test_msm_config_2_20(bellman, oracle):
+    """BASELINE config: n = 2^20, scalars 70% uniform / 20% < 2^64 / 10% in {0,1}."""
+    B, dev = bellman
+    L = 20
+    n = 1 << L
+    s0, d = 0xBC000002, 0x1F2E3D4C5B6A7988
+    bases = oracle.gen_bases_progression(s0, d, n)
+    k = oracle.msm_scalars(0xBC000002, n)
+    db = dev.upload(bases)
+    got = msm(dev, db, k, L)
+    dev.free(db)
+    ks = oracle.from_mont(k)
+    R = oracle.R
+    t = (s0 * sum(ks) + d * sum(i * kv for i, kv in enumerate(ks))) % R
+    assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], oracle.mul_generator(t))

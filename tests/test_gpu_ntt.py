@@ -1,0 +1,78 @@
+"""NTT parity through the C ABI: bit-exact against the CPU oracle and the
+reference's LDE known answers (tests/ntt_test.cu:633-725)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from gpu_helpers import ntt
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def brev(x, bits):
+    return int(format(x, "0%db" % bits)[::-1], 2) if bits else 0
+
+
+@pytest.mark.parametrize("log_n", [1, 2, 5, 8, 11, 12, 13, 16, 19])
+@pytest.mark.parametrize("bitrev,inverse", [(False, False), (True, False), (False, True), (True, True)])
+def test_ntt_matches_oracle(bellman, oracle, log_n, bitrev, inverse):
+    B, dev = bellman
+    x = oracle.random_fr(0xBC000001 + log_n, 1 << log_n)
+    got = ntt(dev, x, log_n, bitrev=bitrev, inverse=inverse)
+    exp = oracle.ntt(x, log_n, bitrev, inverse)
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.parametrize("log_n,log_ext", [(1, 1), (4, 2), (10, 3), (13, 2), (16, 2)])
+@pytest.mark.parametrize("bitrev", [False, True])
+def test_ntt_coset_forward_and_inverse(bellman, oracle, log_n, log_ext, bitrev):
+    B, dev = bellman
+    x = oracle.random_fr(77 + log_n, 1 << log_n)
+    for coset in sorted({0, 1, (1 << log_ext) - 1}):
+        fwd = ntt(dev, x, log_n, bitrev=bitrev, log_ext=log_ext, coset=coset)
+        assert np.array_equal(fwd, oracle.ntt(x, log_n, bitrev, False, log_ext, coset))
+        back = ntt(dev, fwd, log_n, bitrev=not bitrev, inverse=True, log_ext=log_ext, coset=coset)
+        assert np.array_equal(back, x)
+
+
+def test_ntt_lde_known_answers(bellman, oracle):
+    B, dev = bellman
+    kats = json.load(open(os.path.join(GOLDEN, "ntt_lde_kat.json")))
+    for name, remap in (("correctness_lde_2", False), ("correctness_lde_4", True)):
+        k = kats[name]
+        L, e = k["log_values_count"], k["log_extension_degree"]
+        inp = oracle.to_limbs([int(v, 16) for v in k["inputs"]])
+        exp = [int(v, 16) for v in k["expected_outputs"]]
+        for c in range(1 << e):
+            out = oracle.from_limbs(ntt(dev, inp, L, log_ext=e, coset=c))
+            blk = brev(c, e) if remap else c
+            assert out == exp[blk << L:(blk + 1) << L]
+
+
+@pytest.mark.parametrize("host_in,host_out,can_overwrite", [(True, False, False), (False, True, False), (False, True, True), (True, True, False)])
+def test_ntt_host_pointers(bellman, oracle, host_in, host_out, can_overwrite):
+    B, dev = bellman
+    L = 14
+    x = oracle.random_fr(5, 1 << L)
+    got = ntt(dev, x, L, host_in=host_in, host_out=host_out, can_overwrite=can_overwrite)
+    assert np.array_equal(got, oracle.ntt(x, L))
+
+
+def test_ntt_2_24_round_trip_and_sampled_evaluations(bellman, oracle):
+    """Full metric size: forward n2b then inverse b2n is the identity, and
+    sampled outputs equal brute-force polynomial evaluations."""
+    B, dev = bellman
+    L = 24
+    x = oracle.random_fr(0xBC000003, 1 << L)
+    y = ntt(dev, x, L)
+    R = oracle.R
+    w = pow(pow(7, (R - 1) >> 28, R), 1 << (28 - L), R)
+    coeffs = None
+    for k in (0, 1, 12345, (1 << L) - 1):
+        z = oracle.to_mont([pow(w, k, R)])
+        assert np.array_equal(y[brev(k, L)], oracle.poly_eval(x, z)[0])
+    z = ntt(dev, y, L, bitrev=True, inverse=True)
+    assert np.array_equal(z, x)
