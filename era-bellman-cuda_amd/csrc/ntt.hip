@@ -197,14 +197,15 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
   }
 }
 
-// fine[j] = scale * alpha^j (j < 2^12), coarse[j] = alpha^(j << 12)
+// fine[j] = alpha^j (j < 2^12), coarse[j] = scale * alpha^(j << 12), so that
+// twist(i) = fine[i mod 2^12] * coarse[i >> 12] = scale * alpha^i (fine[0] is skipped)
 __global__ void twist_table_kernel(fe *fine, fe *coarse, fe alpha, fe alpha_4096, fe scale, unsigned coarse_count) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned nf = 1u << TWIST_FINE_BITS;
   if (j < nf)
-    store_fe(fine + j, mul<FrParams>(scale, pow_u64<FrParams>(alpha, j)));
+    store_fe(fine + j, pow_u64<FrParams>(alpha, j));
   else if (j - nf < coarse_count)
-    store_fe(coarse + (j - nf), pow_u64<FrParams>(alpha_4096, j - nf));
+    store_fe(coarse + (j - nf), mul<FrParams>(scale, pow_u64<FrParams>(alpha_4096, j - nf)));
 }
 
 struct pass_plan {
