@@ -11,16 +11,19 @@
 //   them as sum = 2*sum + result[i].
 //
 // Pipeline (all on the caller's stream, temporaries from the caller's pool):
-//   1. digits:     one thread per scalar writes W digit words and counts bucket
-//                  sizes with atomics;
-//   2. sort:       exclusive scan of the counts, then an atomic scatter of
-//                  (sign | point index) into bucket order (a counting sort: the
-//                  order inside a bucket is irrelevant to the point sums);
+//   1. digits:     one thread per scalar writes, for each window, the key
+//                  w*2^(c-1) + (digit - 1) (the global bucket id) and the value
+//                  sign | point index;
+//   2. sort:       one stable LSD radix sort of all W*n (key, value) pairs
+//                  (8-bit digits, per-tile LDS histograms, wave-ballot ranking:
+//                  no global atomics, so heavy buckets cost nothing extra),
+//                  then run detection gives each bucket's [start, end);
 //   3. accumulate: buckets are cut into pieces of at most PIECE points; one
 //                  thread accumulates one piece with XYZZ mixed additions, so
 //                  a heavy bucket (e.g. digit 1 of window 0 when many scalars
 //                  are 1) is spread over many threads;
-//   4. fix-up:     multi-piece buckets are combined by a log-depth tree;
+//   4. fix-up:     buckets of up to 8 pieces are summed by one thread; heavier
+//                  buckets are listed and combined by a log-depth tree;
 //   5. per-bit reduction: windows are split recursively into low/high halves
 //                  of their index bits until each holds one bit (the sums are
 //                  split across threads and folded pairwise);
@@ -31,6 +34,7 @@
 #include "curve.cuh"
 #include "msm.h"
 #include "scan.h"
+#include "sort.h"
 
 namespace bc {
 
@@ -67,6 +71,7 @@ constexpr uint32_t SKIP = 0xffffffffu;
 constexpr uint32_t SIGN = 0x80000000u;
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
+constexpr unsigned SPLIT_TERMS = 16;
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,9 +86,13 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-// 1. signed digits; digit word = (d - 1) | sign << 31, or SKIP for d = 0
-__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict__ scalars, unsigned n, unsigned c, unsigned W, uint32_t *__restrict__ digits,
-                                                           uint32_t *__restrict__ counts) {
+// 1. signed digits.  Entry (w, i) gets key = w * 2^(c-1) + (d - 1), i.e. the
+// global bucket id, or NO_BUCKET (sorts after every bucket) for d = 0, and
+// value = sign << 31 | i.
+constexpr uint32_t NO_BUCKET = 0x7fffffffu;
+
+__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict__ scalars, unsigned n, unsigned c, unsigned W, uint32_t *__restrict__ keys,
+                                                           uint32_t *__restrict__ values) {
   using F = FrParams;
   const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n)
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict_
   sub_limbs(neg, fe_const<F>(F::m), pos); // r - s (r for s = 0), as neg<1>
   const uint32_t global_sign = lt(pos, neg) ? 0u : SIGN;
   const fe k = global_sign ? neg : pos;
-  const uint32_t top = 1u << (c - 1), buckets = top;
+  const uint32_t top = 1u << (c - 1);
   uint32_t borrow = 0;
   for (unsigned w = 0; w < W; w++) {
     uint32_t d = extract_bits(k, w * c, c) + borrow;
@@ -104,34 +113,35 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict_
       borrow = 1;
       sign ^= SIGN;
     }
-    uint32_t word = SKIP;
-    if (d) {
-      word = (d - 1) | sign;
-      atomicAdd(&counts[w * buckets + d - 1], 1u);
-    }
-    digits[(size_t)w * n + i] = word;
+    const size_t e = (size_t)w * n + i;
+    keys[e] = d ? w * top + (d - 1) : NO_BUCKET;
+    values[e] = sign | i;
   }
 }
 
-// 2. scatter (sign | point index) into bucket order
-__global__ __launch_bounds__(MSM_BLOCK) void scatter_kernel(const uint32_t *__restrict__ digits, unsigned n, unsigned W, unsigned buckets,
-                                                            uint32_t *__restrict__ cursor, uint32_t *__restrict__ sorted) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (size_t)W * n)
+// 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted arrays
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const uint32_t *__restrict__ keys, size_t entries, uint32_t *__restrict__ start,
+                                                                uint32_t *__restrict__ end) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= entries)
     return;
-  const uint32_t word = digits[t];
-  if (word == SKIP)
+  const uint32_t k = keys[e];
+  if (k == NO_BUCKET)
     return;
-  const unsigned w = (unsigned)(t / n), i = (unsigned)(t % n);
-  const uint32_t b = w * buckets + (word & ~SIGN);
-  const uint32_t pos = atomicAdd(&cursor[b], 1u);
-  sorted[pos] = (word & SIGN) | i;
+  if (e == 0 || keys[e - 1] != k)
+    start[k] = (uint32_t)e;
+  if (e + 1 == entries || keys[e + 1] != k)
+    end[k] = (uint32_t)(e + 1);
 }
 
-__global__ void piece_count_kernel(const uint32_t *counts, uint32_t *pieces, unsigned nb, unsigned piece) {
+// counts[b] = end[b] - start[b] (in place over `end`), pieces of at most `piece` points
+__global__ void piece_count_kernel(const uint32_t *start, uint32_t *counts, uint32_t *pieces, unsigned nb, unsigned piece) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nb)
-    pieces[b] = (counts[b] + piece - 1) / piece;
+  if (b >= nb)
+    return;
+  const uint32_t cnt = counts[b] - start[b];
+  counts[b] = cnt;
+  pieces[b] = (cnt + piece - 1) / piece;
 }
 
 __global__ void piece_map_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket, unsigned nb, uint32_t *total) {
@@ -169,17 +179,76 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const uint32_t *_
   store_xyzz(partials + p, acc);
 }
 
-// 4. tree step: piece q of its bucket absorbs piece q + stride when q % (2 stride) == 0
-__global__ __launch_bounds__(ACC_BLOCK) void piece_reduce_kernel(g1_xyzz *partials, const uint32_t *piece_bucket, const uint32_t *piece_offset,
-                                                                 const uint32_t *piece_count, const uint32_t *total, unsigned stride) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= *total)
+// 4a. buckets of 2..SMALL_PIECES pieces are summed serially into piece 0;
+// larger ("heavy") buckets are listed for the tree below.
+constexpr unsigned SMALL_PIECES = 8;
+__global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count, unsigned nb,
+                                                                uint32_t *heavy, uint32_t *heavy_count) {
+  const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb)
     return;
-  const uint32_t b = piece_bucket[p];
-  const uint32_t q = p - piece_offset[b];
-  if ((q & (2 * stride - 1)) != 0 || q + stride >= piece_count[b])
+  const uint32_t np = piece_count[b];
+  if (np < 2)
     return;
-  store_xyzz(partials + p, xyzz_add(load_xyzz(partials + p), load_xyzz(partials + p + stride)));
+  if (np > SMALL_PIECES) {
+    heavy[atomicAdd(heavy_count, 1u)] = b;
+    return;
+  }
+  const uint32_t off = piece_offset[b];
+  g1_xyzz acc = load_xyzz(partials + off);
+  for (uint32_t q = 1; q < np; q++)
+    acc = xyzz_add(acc, load_xyzz(partials + off + q));
+  store_xyzz(partials + off, acc);
+}
+
+// exclusive prefix of the heavy buckets' piece counts (one block, chunked)
+__global__ __launch_bounds__(256) void heavy_prefix_kernel(const uint32_t *heavy, const uint32_t *heavy_count, const uint32_t *piece_count,
+                                                           uint32_t *heavy_offset, uint32_t *heavy_total) {
+  __shared__ uint32_t sh[256];
+  const uint32_t count = *heavy_count;
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < count; base += 256) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < count ? piece_count[heavy[i]] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (unsigned off = 1; off < 256; off <<= 1) {
+      const uint32_t t = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < count)
+      heavy_offset[i] = carry + sh[threadIdx.x] - v;
+    carry += sh[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    *heavy_total = carry;
+}
+
+// 4b. one tree level over the pieces of heavy buckets: piece q absorbs q + stride when q % (2 stride) == 0
+__global__ __launch_bounds__(ACC_BLOCK) void heavy_tree_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+                                                               const uint32_t *heavy_offset, const uint32_t *heavy_total, const uint32_t *piece_offset,
+                                                               const uint32_t *piece_count, unsigned stride) {
+  const uint32_t total = *heavy_total, count = *heavy_count;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
+    // heavy bucket holding flattened piece h: last j with heavy_offset[j] <= h
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (heavy_offset[mid] <= h)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const uint32_t b = heavy[lo];
+    const uint32_t q = h - heavy_offset[lo];
+    if ((q & (2 * stride - 1)) != 0 || q + stride >= piece_count[b])
+      continue;
+    g1_xyzz *p = partials + piece_offset[b] + q;
+    store_xyzz(p, xyzz_add(load_xyzz(p), load_xyzz(p + stride)));
+  }
 }
 
 // level-0 bucket array A[w][k], k in [0, 2^(c-1)); tops[w] = bucket of the top digit
@@ -210,12 +279,14 @@ __global__ void bucket_gather_kernel(const g1_xyzz *partials, const uint32_t *pi
 }
 
 // 5. split: windows of width s (2^s buckets) -> 2X windows of width t = ceil(s/2);
-// lo[u] = sum_v A[v*2^t + u], hi[v] = sum_u A[v*2^t + u]; thread sums the terms == part (mod D)
-__global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const g1_xyzz *A, unsigned X, unsigned s, unsigned D, g1_xyzz *out) {
+// lo[u] = sum_v A[v*2^t + u] (2^hs terms), hi[v] = sum_u A[v*2^t + u] (2^t terms).
+// Output part p holds the terms == p (mod D_lo) of lo and (mod D_hi) of hi, with
+// D_hi = D_lo * 2^(t-hs) so every thread sums the same number of terms.
+__global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const g1_xyzz *A, unsigned X, unsigned s, unsigned D_lo, unsigned D_hi, g1_xyzz *out) {
   const unsigned t = (s + 1) >> 1, hs = s - t;
   const unsigned per_part = (2 * X) << t;
   const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= D * per_part)
+  if (id >= D_hi * per_part)
     return;
   const unsigned part = id / per_part, rem = id % per_part;
   const unsigned xo = rem >> t, j = rem & ((1u << t) - 1);
@@ -223,10 +294,11 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const g1_xyzz *A, unsi
   const g1_xyzz *src = A + ((size_t)x << s);
   g1_xyzz acc = xyzz_infinity();
   if (half == 0) {
-    for (unsigned v = part; v < (1u << hs); v += D)
-      acc = xyzz_add(acc, load_xyzz(src + ((v << t) | j)));
+    if (part < D_lo)
+      for (unsigned v = part; v < (1u << hs); v += D_lo)
+        acc = xyzz_add(acc, load_xyzz(src + ((v << t) | j)));
   } else if (j < (1u << hs)) {
-    for (unsigned u = part; u < (1u << t); u += D)
+    for (unsigned u = part; u < (1u << t); u += D_hi)
       acc = xyzz_add(acc, load_xyzz(src + ((j << t) | u)));
   }
   store_xyzz(out + id, acc);
@@ -279,7 +351,7 @@ template <class T> hipError_t alloc(T **p, size_t n, hipMemPool_t pool, hipStrea
 // results (device, 254 Jacobian).
 hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n, unsigned c, unsigned piece, g1_jacobian *results, hipMemPool_t pool,
                       hipStream_t s) {
-  if (c < 3 || c > 24 || piece == 0)
+  if (c < 3 || c > 22 || piece == 0)
     return hipErrorInvalidValue;
   const unsigned n = 1u << log_n;
   const unsigned W = (254 - 1) / c + 1;
@@ -289,13 +361,11 @@ hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n,
   const size_t entries = (size_t)W * n;
   const size_t max_pieces = entries / piece + nb + 1;
 
-  uint32_t *digits = nullptr, *counts = nullptr, *offsets = nullptr, *cursor = nullptr, *sorted = nullptr;
+  uint32_t *counts = nullptr, *offsets = nullptr, *sorted = nullptr;
   uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *total = nullptr;
   g1_xyzz *partials = nullptr;
-  BC_TRY(alloc(&digits, entries, pool, s));
   BC_TRY(alloc(&counts, nb, pool, s));
   BC_TRY(alloc(&offsets, nb, pool, s));
-  BC_TRY(alloc(&cursor, nb, pool, s));
   BC_TRY(alloc(&sorted, entries, pool, s));
   BC_TRY(alloc(&piece_count, nb, pool, s));
   BC_TRY(alloc(&piece_offset, nb, pool, s));
@@ -303,28 +373,53 @@ hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n,
   BC_TRY(alloc(&total, 1, pool, s));
   BC_TRY(alloc(&partials, max_pieces, pool, s));
 
-  BC_TRY(fill_u32(counts, nb, 0, s));
-  digits_kernel<<<grid_for(n, MSM_BLOCK), MSM_BLOCK, 0, s>>>(scalars, n, c, W, digits, counts);
-  BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(counts, offsets, nb, pool, s));
-  BC_TRY(hipMemcpyAsync(cursor, offsets, sizeof(uint32_t) * nb, hipMemcpyDeviceToDevice, s));
-  scatter_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(digits, n, W, B, cursor, sorted);
-  BC_LAUNCH_CHECK();
-  piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(counts, piece_count, nb, piece);
+  {
+    uint32_t *keys = nullptr, *vals = nullptr, *sorted_keys = nullptr;
+    BC_TRY(alloc(&keys, entries, pool, s));
+    BC_TRY(alloc(&vals, entries, pool, s));
+    BC_TRY(alloc(&sorted_keys, entries, pool, s));
+    digits_kernel<<<grid_for(n, MSM_BLOCK), MSM_BLOCK, 0, s>>>(scalars, n, c, W, keys, vals);
+    BC_LAUNCH_CHECK();
+    unsigned key_bits = 0;
+    while ((1ull << key_bits) <= (uint64_t)W << (c - 1))
+      key_bits++;
+    // NO_BUCKET has every bit set below bit 31: it sorts after all buckets on key_bits
+    BC_TRY(radix_sort_pairs(keys, sorted_keys, vals, sorted, entries, 0, key_bits, pool, s));
+    BC_TRY(fill_u32(offsets, nb, 0, s));
+    BC_TRY(fill_u32(counts, nb, 0, s));
+    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(sorted_keys, entries, offsets, counts);
+    BC_LAUNCH_CHECK();
+    BC_TRY(pool_free(keys, s));
+    BC_TRY(pool_free(vals, s));
+    BC_TRY(pool_free(sorted_keys, s));
+  }
+  piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(offsets, counts, piece_count, nb, piece);
   BC_LAUNCH_CHECK();
   BC_TRY(scan_u32_exclusive(piece_count, piece_offset, nb, pool, s));
   piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, total);
   BC_LAUNCH_CHECK();
   accumulate_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(sorted, offsets, counts, piece_offset, piece_bucket, total, bases, piece, partials);
   BC_LAUNCH_CHECK();
+  uint32_t *heavy = nullptr, *heavy_offset = nullptr, *heavy_meta = nullptr; // meta: count, total
+  BC_TRY(alloc(&heavy, nb, pool, s));
+  BC_TRY(alloc(&heavy_offset, nb, pool, s));
+  BC_TRY(alloc(&heavy_meta, 2, pool, s));
+  BC_TRY(fill_u32(heavy_meta, 2, 0, s));
+  small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, heavy_meta);
+  BC_LAUNCH_CHECK();
   const unsigned max_np = (n + piece - 1) / piece;
-  for (unsigned stride = 1; stride < max_np; stride <<= 1) {
-    piece_reduce_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_bucket, piece_offset, piece_count, total, stride);
+  if (max_np > SMALL_PIECES) {
+    heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, heavy_meta, piece_count, heavy_offset, heavy_meta + 1);
     BC_LAUNCH_CHECK();
+    for (unsigned stride = 1; stride < max_np; stride <<= 1) {
+      heavy_tree_kernel<<<1024, ACC_BLOCK, 0, s>>>(partials, heavy, heavy_meta, heavy_offset, heavy_meta + 1, piece_offset, piece_count, stride);
+      BC_LAUNCH_CHECK();
+    }
   }
-  BC_TRY(pool_free(digits, s));
+  BC_TRY(pool_free(heavy, s));
+  BC_TRY(pool_free(heavy_offset, s));
+  BC_TRY(pool_free(heavy_meta, s));
   BC_TRY(pool_free(sorted, s));
-  BC_TRY(pool_free(cursor, s));
 
   // level-0 windows
   g1_xyzz *A = nullptr, *tops = nullptr;
@@ -345,12 +440,12 @@ hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n,
   while (width > 1) {
     const unsigned t = (width + 1) / 2, hs = width - t;
     const unsigned per_part = (2 * X) << t;
-    unsigned D = 1;
-    while (D < (1u << hs) && (size_t)(2 * D) * per_part <= (1u << 18))
-      D <<= 1;
+    // about SPLIT_TERMS terms per thread
+    const unsigned D_lo = (1u << hs) > SPLIT_TERMS ? (1u << hs) / SPLIT_TERMS : 1u;
+    const unsigned D = D_lo << (t - hs);
     g1_xyzz *next = nullptr;
     BC_TRY(alloc(&next, (size_t)D * per_part, pool, s));
-    split_kernel<<<grid_for((size_t)D * per_part, ACC_BLOCK), ACC_BLOCK, 0, s>>>(A, X, width, D, next);
+    split_kernel<<<grid_for((size_t)D * per_part, ACC_BLOCK), ACC_BLOCK, 0, s>>>(A, X, width, D_lo, D, next);
     BC_LAUNCH_CHECK();
     for (unsigned d = D / 2; d >= 1; d /= 2) {
       fold_parts_kernel<<<grid_for((size_t)d * per_part, ACC_BLOCK), ACC_BLOCK, 0, s>>>(next, d * per_part);

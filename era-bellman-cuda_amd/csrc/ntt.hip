@@ -147,13 +147,17 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
   }
   __syncthreads();
 
-  const unsigned bflies = elems >> 1;
+  // Stages in network order (CT: s_lo..s_hi-1, GS: s_hi-1..s_lo).  An odd
+  // count starts with one radix-2 step; the rest go two at a time: a thread
+  // holds the 4 elements t0, t0+h/2, t0+h, t0+3h/2 (h = row distance of the
+  // lower stage s) and applies stages s and s+1 in registers.
   const unsigned nst = a.s_hi - a.s_lo;
-  for (unsigned k = 0; k < nst; k++) {
-    const unsigned s = GS ? a.s_hi - 1 - k : a.s_lo + k;
-    const unsigned log_h = a.s_hi - s - 1; // row distance (log)
+  unsigned done = 0;
+  if (nst & 1) {
+    const unsigned s = GS ? a.s_hi - 1 : a.s_lo;
+    const unsigned log_h = a.s_hi - s - 1;
     const unsigned h = 1u << log_h;
-    for (unsigned q = threadIdx.x; q < bflies; q += NTT_BLOCK) {
+    for (unsigned q = threadIdx.x; q < (elems >> 1); q += NTT_BLOCK) {
       const unsigned c = q & cmask;
       const unsigned r = (q >> a.log_c) & (tmask >> 1);
       const unsigned g = q >> (a.log_c + a.log_t - 1);
@@ -161,16 +165,62 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
       const unsigned i0 = (((g << a.log_t) | t0) << a.log_c) | c;
       const unsigned i1 = i0 + (h << a.log_c);
       const unsigned b = ((hi0 + g) << (s - a.s_lo)) | (t0 >> (a.s_hi - s));
-      fe E = tile[i0], O = tile[i1];
+      const fe E = tile[i0], O = tile[i1];
       if (!GS) {
-        if (s != 0)
-          O = mul_lazy<F>(O, load_fe(a.tw + (1u << s) + b));
-        tile[i0] = add2<F>(E, O);
-        tile[i1] = sub2<F>(E, O);
+        const fe wo = s != 0 ? mul_lazy<F>(O, load_fe(a.tw + (1u << s) + b)) : O;
+        tile[i0] = add2<F>(E, wo);
+        tile[i1] = sub2<F>(E, wo);
       } else {
         const fe d = sub2<F>(E, O);
         tile[i0] = add2<F>(E, O);
         tile[i1] = s != 0 ? mul_lazy<F>(d, load_fe(a.tw + (1u << s) + b)) : d;
+      }
+    }
+    __syncthreads();
+    done = 1;
+  }
+  for (; done < nst; done += 2) {
+    // the lower-numbered stage of the pair
+    const unsigned s = GS ? a.s_hi - 2 - done : a.s_lo + done;
+    const unsigned log_h = a.s_hi - s - 1; // >= 1
+    const unsigned h = 1u << log_h, hh = h >> 1;
+    for (unsigned q = threadIdx.x; q < (elems >> 2); q += NTT_BLOCK) {
+      const unsigned c = q & cmask;
+      const unsigned u = (q >> a.log_c) & (tmask >> 2);
+      const unsigned g = q >> (a.log_c + a.log_t - 2);
+      const unsigned t0 = ((u >> (log_h - 1)) << (log_h + 1)) | (u & (hh - 1));
+      const unsigned i0 = (((g << a.log_t) | t0) << a.log_c) | c;
+      const unsigned i1 = i0 + (hh << a.log_c), i2 = i0 + (h << a.log_c), i3 = i2 + (hh << a.log_c);
+      const unsigned b = ((hi0 + g) << (s - a.s_lo)) | (t0 >> (a.s_hi - s));
+      const fe *tw_s1 = a.tw + (2u << s) + 2 * b; // stage s+1 twiddles of blocks 2b, 2b+1
+      fe x0 = tile[i0], x1 = tile[i1], x2 = tile[i2], x3 = tile[i3];
+      if (!GS) {
+        if (s != 0) {
+          const fe wa = load_fe(a.tw + (1u << s) + b);
+          x2 = mul_lazy<F>(x2, wa);
+          x3 = mul_lazy<F>(x3, wa);
+        }
+        fe y0 = add2<F>(x0, x2), y2 = sub2<F>(x0, x2);
+        fe y1 = add2<F>(x1, x3), y3 = sub2<F>(x1, x3);
+        y1 = mul_lazy<F>(y1, load_fe(tw_s1));
+        y3 = mul_lazy<F>(y3, load_fe(tw_s1 + 1));
+        tile[i0] = add2<F>(y0, y1);
+        tile[i1] = sub2<F>(y0, y1);
+        tile[i2] = add2<F>(y2, y3);
+        tile[i3] = sub2<F>(y2, y3);
+      } else {
+        const fe y0 = add2<F>(x0, x1), y1 = mul_lazy<F>(sub2<F>(x0, x1), load_fe(tw_s1));
+        const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub2<F>(x2, x3), load_fe(tw_s1 + 1));
+        fe z2 = sub2<F>(y0, y2), z3 = sub2<F>(y1, y3);
+        if (s != 0) {
+          const fe wa = load_fe(a.tw + (1u << s) + b);
+          z2 = mul_lazy<F>(z2, wa);
+          z3 = mul_lazy<F>(z3, wa);
+        }
+        tile[i0] = add2<F>(y0, y2);
+        tile[i1] = add2<F>(y1, y3);
+        tile[i2] = z2;
+        tile[i3] = z3;
       }
     }
     __syncthreads();
