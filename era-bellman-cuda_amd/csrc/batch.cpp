@@ -1,0 +1,470 @@
+// Batched execution, tuning and hipGraph plans (extension API of
+// include/bellman-amd.h; no counterpart in the reference).
+//
+// A batch runs every phase of all its jobs as one kernel launch (the job tables
+// travel in the kernel arguments), so many small MSMs or NTTs cost one launch
+// sequence instead of one per job.  A plan captures the launch sequence of a
+// fixed job set once into a hipGraph whose temporaries live in a workspace
+// owned by the plan; replaying it costs one graph launch.
+#include <algorithm>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+#include "common.h"
+#include "tuning.h"
+
+namespace bc {
+
+// ---------------------------------------------------------------- tuning
+
+namespace {
+
+constexpr unsigned TUNE_MAX_LOG = 28;
+
+struct tuning_table {
+  msm_tuning msm[TUNE_MAX_LOG + 1];
+  unsigned ntt_first[TUNE_MAX_LOG + 1];
+  tuning_table() {
+    for (unsigned l = 0; l <= TUNE_MAX_LOG; l++) {
+      msm[l] = {msm_window_bits(l), MSM_DEFAULT_PIECE};
+      ntt_first[l] = 0;
+    }
+  }
+};
+
+std::mutex g_tune_mutex;
+tuning_table &tune() {
+  static tuning_table t;
+  return t;
+}
+
+} // namespace
+
+msm_tuning msm_tuned(unsigned log_n) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  return tune().msm[std::min(log_n, TUNE_MAX_LOG)];
+}
+
+void msm_set_tuned(unsigned log_n, msm_tuning t) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (log_n <= TUNE_MAX_LOG)
+    tune().msm[log_n] = t;
+}
+
+unsigned ntt_first_pass(unsigned log_n) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  return log_n <= TUNE_MAX_LOG ? tune().ntt_first[log_n] : 0;
+}
+
+void ntt_set_first_pass(unsigned log_n, unsigned stages) {
+  std::lock_guard<std::mutex> lock(g_tune_mutex);
+  if (log_n <= TUNE_MAX_LOG)
+    tune().ntt_first[log_n] = stages;
+}
+
+// ---------------------------------------------------------------- jobs
+
+hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out) {
+  if (in.log_scalars_count > 28 || !in.bases || !in.scalars || !in.results)
+    return hipErrorInvalidValue;
+  const msm_tuning t = msm_tuned(in.log_scalars_count);
+  out->bases = static_cast<const g1_affine *>(in.bases);
+  out->scalars = static_cast<const fe *>(in.scalars);
+  out->results = static_cast<g1_jacobian *>(in.results);
+  out->log_n = in.log_scalars_count;
+  out->c = in.window_bits ? in.window_bits : t.c;
+  out->piece = in.piece ? in.piece : t.piece;
+  if (out->c < 3 || out->c > 24)
+    return hipErrorInvalidValue;
+  return hipSuccess;
+}
+
+ntt_job resolve_ntt(const bc_amd_ntt_job &in) {
+  return ntt_job{static_cast<const fe *>(in.inputs),   static_cast<fe *>(in.outputs), in.log_values_count, in.bit_reversed_inputs,
+                 in.inverse, in.log_extension_degree, in.coset_index};
+}
+
+// Runs MSM jobs in launches of at most MSM_MAX_JOBS jobs and 2^31 - 1 digit entries.
+hipError_t msm_run(const std::vector<msm_job> &jobs, arena &ar) {
+  size_t i = 0;
+  while (i < jobs.size()) {
+    size_t j = i;
+    uint64_t entries = 0;
+    while (j < jobs.size() && j - i < MSM_MAX_JOBS) {
+      const uint64_t e = (uint64_t)((254 - 1) / jobs[j].c + 1) << jobs[j].log_n;
+      if (j > i && entries + e >= (1ull << 31))
+        break;
+      entries += e;
+      j++;
+    }
+    BC_TRY(msm_batch_device(jobs.data() + i, (unsigned)(j - i), ar));
+    i = j;
+  }
+  return hipSuccess;
+}
+
+hipError_t ntt_run(const std::vector<ntt_job> &jobs, arena &ar) {
+  for (size_t i = 0; i < jobs.size(); i += NTT_MAX_JOBS)
+    BC_TRY(ntt_batch_device(jobs.data() + i, (unsigned)std::min<size_t>(NTT_MAX_JOBS, jobs.size() - i), ar));
+  return hipSuccess;
+}
+
+// Temporaries of msm_batch_device for one job: 6 x 4 B per digit entry (keys,
+// values, their sorted copies, the sort's ping-pong buffers), the sort's tile
+// histograms, 132 B per piece (partial sum + bucket id), and per bucket the
+// run/piece tables, the level-0 window array and the split levels (< 0.6 of it).
+size_t msm_footprint(const msm_job &j) {
+  const uint64_t W = (254 - 1) / j.c + 1;
+  const uint64_t E = W << j.log_n, nb = W << (j.c - 1);
+  return (size_t)(25 * E + 132 * (E / j.piece + nb + 1) + 24 * nb + 128 * nb * 8 / 5 + (1u << 20));
+}
+
+// Coset transforms hold one twist table (2^12 fine + 2^(L-12) coarse entries).
+size_t ntt_footprint(const ntt_job &j) {
+  const uint64_t coarse = j.log_n > 12 ? 1ull << (j.log_n - 12) : 1;
+  return j.log_ext ? (size_t)(32 * (4096 + coarse) + 4096) : 4096;
+}
+
+// pn_generate_device: keys, cells, sorted copies, flags, scan, run tables
+// (11 x 4 B per cell), the sort's ping-pong buffers (8 B) and histograms.
+size_t pn_footprint(unsigned cols, unsigned log_rows) {
+  const uint64_t n = (uint64_t)cols << log_rows;
+  return (size_t)(53 * n + (1u << 20));
+}
+
+// ---------------------------------------------------------------- autotune
+
+namespace {
+
+// Median of three timed runs after one warm-up, in milliseconds.
+template <class F> hipError_t time_median(hipStream_t s, F run, float *ms) {
+  hipEvent_t a, b;
+  BC_TRY(hipEventCreate(&a));
+  BC_TRY(hipEventCreate(&b));
+  float t[3];
+  hipError_t e = run();
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    e = hipEventRecord(a, s);
+    if (e == hipSuccess)
+      e = run();
+    if (e == hipSuccess)
+      e = hipEventRecord(b, s);
+    if (e == hipSuccess)
+      e = hipEventSynchronize(b);
+    if (e == hipSuccess)
+      e = hipEventElapsedTime(&t[k], a, b);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  BC_TRY(e);
+  std::sort(t, t + 3);
+  *ms = t[1];
+  return hipSuccess;
+}
+
+hipError_t msm_autotune(const void *bases, const void *scalars, unsigned log_min, unsigned log_max, hipStream_t s) {
+  if (log_min > log_max || log_max > 26 || !bases || !scalars)
+    return hipErrorInvalidValue;
+  g1_jacobian *results = nullptr;
+  BC_TRY(hipMalloc(&results, sizeof(g1_jacobian) * 254));
+  hipError_t status = hipSuccess;
+  for (unsigned L = log_min; L <= log_max && status == hipSuccess; L++) {
+    auto timed = [&](unsigned c, unsigned piece, float *ms) {
+      const msm_job job{static_cast<const g1_affine *>(bases), static_cast<const fe *>(scalars), results, L, c, piece};
+      return time_median(
+          s,
+          [&]() {
+            arena ar = arena::pool_mode(nullptr, s);
+            return msm_batch_device(&job, 1, ar);
+          },
+          ms);
+    };
+    const unsigned c0 = msm_window_bits(L);
+    const unsigned c_lo = std::max(4u, c0 > 6 ? c0 - 6 : 4u), c_hi = std::min(22u, c0 + 2);
+    unsigned best_c = c0;
+    float best = 1e30f;
+    for (unsigned c = c_lo; c <= c_hi && status == hipSuccess; c++) {
+      float ms = 0;
+      status = timed(c, MSM_DEFAULT_PIECE, &ms);
+      if (status == hipSuccess && ms < best) {
+        best = ms;
+        best_c = c;
+      }
+    }
+    unsigned best_piece = MSM_DEFAULT_PIECE;
+    for (unsigned piece : {8u, 16u, 64u, 128u}) {
+      if (status != hipSuccess)
+        break;
+      float ms = 0;
+      status = timed(best_c, piece, &ms);
+      if (status == hipSuccess && ms < best) {
+        best = ms;
+        best_piece = piece;
+      }
+    }
+    if (status == hipSuccess)
+      msm_set_tuned(L, {best_c, best_piece});
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(results);
+  return status;
+}
+
+hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipStream_t s) {
+  if (log_min > log_max || log_max > NTT_MAX_LOG || !scratch)
+    return hipErrorInvalidValue;
+  BC_TRY(ntt_prepare(std::min(log_max, 24u), s, nullptr));
+  fe *buf = static_cast<fe *>(scratch);
+  for (unsigned L = std::max(log_min, 13u); L <= std::min(log_max, 24u); L++) {
+    unsigned best_first = 0;
+    float best = 1e30f;
+    for (unsigned first = L - 12; first <= 12; first++) {
+      ntt_set_first_pass(L, first);
+      float ms = 0;
+      BC_TRY(time_median(
+          s,
+          [&]() {
+            arena ar = arena::pool_mode(nullptr, s);
+            const ntt_job pair[2] = {{buf, buf, L, false, false, 0, 0}, {buf, buf, L, true, true, 0, 0}};
+            BC_TRY(ntt_batch_device(&pair[0], 1, ar));
+            return ntt_batch_device(&pair[1], 1, ar);
+          },
+          &ms));
+      if (ms < best) {
+        best = ms;
+        best_first = first;
+      }
+    }
+    ntt_set_first_pass(L, best_first);
+  }
+  return hipStreamSynchronize(s);
+}
+
+} // namespace
+
+// ---------------------------------------------------------------- plans
+
+namespace {
+
+struct plan_state {
+  std::vector<msm_job> msm;
+  std::vector<ntt_job> ntt;
+  void *workspace = nullptr;
+  size_t bytes = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t last = nullptr;
+};
+
+hipError_t record(const plan_state &p, arena &ar) {
+  BC_TRY(msm_run(p.msm, ar));
+  return ntt_run(p.ntt, ar);
+}
+
+// Captures the plan's launch sequence on `s` with arena `ar`.
+hipError_t capture(const plan_state &p, arena &ar, hipStream_t s, hipGraph_t *graph) {
+  BC_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = record(p, ar);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (e != hipSuccess || end != hipSuccess) {
+    if (g)
+      (void)hipGraphDestroy(g);
+    return e != hipSuccess ? e : end;
+  }
+  *graph = g;
+  return hipSuccess;
+}
+
+hipError_t plan_create(plan_state *p) {
+  unsigned max_log = 0;
+  for (const ntt_job &j : p->ntt)
+    max_log = std::max(max_log, j.log_n);
+  hipStream_t s = nullptr;
+  BC_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipError_t e = max_log ? ntt_prepare(max_log, s, nullptr) : hipSuccess;
+  // First capture only sizes the workspace: its kernels see placeholder
+  // addresses and the graph is destroyed without being launched.
+  if (e == hipSuccess) {
+    arena sizing = arena::bump_mode(reinterpret_cast<void *>(uintptr_t(1) << 44), size_t(1) << 43, s);
+    hipGraph_t g = nullptr;
+    e = capture(*p, sizing, s, &g);
+    if (g)
+      (void)hipGraphDestroy(g);
+    p->bytes = sizing.requested();
+  }
+  if (e == hipSuccess)
+    e = hipMalloc(&p->workspace, std::max<size_t>(p->bytes, 256));
+  if (e == hipSuccess) {
+    arena ws = arena::bump_mode(p->workspace, p->bytes, s);
+    e = capture(*p, ws, s, &p->graph);
+  }
+  if (e == hipSuccess)
+    e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&p->last, hipEventDisableTiming);
+  (void)hipStreamDestroy(s);
+  return e;
+}
+
+void plan_free(plan_state *p) {
+  if (p->last) {
+    (void)hipEventSynchronize(p->last);
+    (void)hipEventDestroy(p->last);
+  }
+  if (p->exec)
+    (void)hipGraphExecDestroy(p->exec);
+  if (p->graph)
+    (void)hipGraphDestroy(p->graph);
+  if (p->workspace)
+    (void)hipFree(p->workspace);
+  delete p;
+}
+
+} // namespace
+
+} // namespace bc
+
+using namespace bc;
+
+extern "C" {
+
+bc_error bc_amd_msm_batch_async(const bc_amd_msm_job *jobs, unsigned count, bc_mem_pool pool, bc_stream stream) {
+  if (count && !jobs)
+    return bc_error_invalid_value;
+  std::vector<msm_job> v(count);
+  for (unsigned i = 0; i < count; i++)
+    BC_TRY_BC(resolve_msm(jobs[i], &v[i]));
+  arena ar = arena::pool_mode(to_pool(pool), to_stream(stream));
+  return as_bc(msm_run(v, ar));
+}
+
+bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_mem_pool pool, bc_stream stream) {
+  if (count && !jobs)
+    return bc_error_invalid_value;
+  std::vector<ntt_job> v(count);
+  for (unsigned i = 0; i < count; i++)
+    v[i] = resolve_ntt(jobs[i]);
+  arena ar = arena::pool_mode(to_pool(pool), to_stream(stream));
+  return as_bc(ntt_run(v, ar));
+}
+
+bc_error bc_amd_msm_get_tuning(unsigned log_scalars_count, bc_amd_msm_tuning *tuning) {
+  if (!tuning || log_scalars_count > 28)
+    return bc_error_invalid_value;
+  const msm_tuning t = msm_tuned(log_scalars_count);
+  tuning->window_bits = t.c;
+  tuning->piece = t.piece;
+  return bc_success;
+}
+
+bc_error bc_amd_msm_set_tuning(unsigned log_scalars_count, bc_amd_msm_tuning tuning) {
+  if (log_scalars_count > 28 || tuning.window_bits < 3 || tuning.window_bits > 24 || tuning.piece == 0)
+    return bc_error_invalid_value;
+  msm_set_tuned(log_scalars_count, {tuning.window_bits, tuning.piece});
+  return bc_success;
+}
+
+bc_error bc_amd_ntt_get_split(unsigned log_values_count, unsigned *first_pass_stages) {
+  if (!first_pass_stages || log_values_count > 28)
+    return bc_error_invalid_value;
+  *first_pass_stages = ntt_first_pass(log_values_count);
+  return bc_success;
+}
+
+bc_error bc_amd_ntt_set_split(unsigned log_values_count, unsigned first_pass_stages) {
+  const unsigned L = log_values_count;
+  if (L > 28 || (first_pass_stages && (L < 13 || L > 24 || first_pass_stages > 12 || L - first_pass_stages > 12)))
+    return bc_error_invalid_value;
+  ntt_set_first_pass(L, first_pass_stages);
+  return bc_success;
+}
+
+bc_error bc_amd_msm_autotune(const void *bases, const void *scalars, unsigned log_min, unsigned log_max, bc_stream stream) {
+  return as_bc(msm_autotune(bases, scalars, log_min, log_max, to_stream(stream)));
+}
+
+bc_error bc_amd_ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, bc_stream stream) {
+  return as_bc(ntt_autotune(scratch, log_min, log_max, to_stream(stream)));
+}
+
+bc_error bc_amd_tuning_save(const char *path) {
+  FILE *f = path ? std::fopen(path, "w") : nullptr;
+  if (!f)
+    return bc_error_invalid_value;
+  for (unsigned l = 0; l <= TUNE_MAX_LOG; l++) {
+    const msm_tuning t = msm_tuned(l);
+    std::fprintf(f, "msm %u %u %u\n", l, t.c, t.piece);
+  }
+  for (unsigned l = 13; l <= 24; l++)
+    std::fprintf(f, "ntt %u %u\n", l, ntt_first_pass(l));
+  return std::fclose(f) == 0 ? bc_success : bc_error_invalid_value;
+}
+
+bc_error bc_amd_tuning_load(const char *path) {
+  FILE *f = path ? std::fopen(path, "r") : nullptr;
+  if (!f)
+    return bc_error_invalid_value;
+  char kind[8];
+  bc_error status = bc_success;
+  while (status == bc_success && std::fscanf(f, "%7s", kind) == 1) {
+    unsigned l = 0, a = 0, b = 0;
+    if (std::string(kind) == "msm" && std::fscanf(f, "%u %u %u", &l, &a, &b) == 3)
+      status = bc_amd_msm_set_tuning(l, {a, b});
+    else if (std::string(kind) == "ntt" && std::fscanf(f, "%u %u", &l, &a) == 2)
+      status = bc_amd_ntt_set_split(l, a);
+    else
+      status = bc_error_invalid_value;
+  }
+  std::fclose(f);
+  return status;
+}
+
+bc_error bc_amd_plan_create(bc_amd_plan *plan, const bc_amd_msm_job *msm_jobs, unsigned msm_count, const bc_amd_ntt_job *ntt_jobs, unsigned ntt_count) {
+  if (!plan || (msm_count && !msm_jobs) || (ntt_count && !ntt_jobs))
+    return bc_error_invalid_value;
+  plan_state *p = new plan_state;
+  p->msm.resize(msm_count);
+  p->ntt.resize(ntt_count);
+  hipError_t e = hipSuccess;
+  for (unsigned i = 0; i < msm_count && e == hipSuccess; i++)
+    e = resolve_msm(msm_jobs[i], &p->msm[i]);
+  for (unsigned i = 0; i < ntt_count; i++)
+    p->ntt[i] = resolve_ntt(ntt_jobs[i]);
+  if (e == hipSuccess)
+    e = plan_create(p);
+  if (e != hipSuccess) {
+    plan_free(p);
+    plan->handle = nullptr;
+    return as_bc(e);
+  }
+  plan->handle = p;
+  return bc_success;
+}
+
+bc_error bc_amd_plan_workspace_bytes(bc_amd_plan plan, size_t *bytes) {
+  if (!plan.handle || !bytes)
+    return bc_error_invalid_value;
+  *bytes = static_cast<plan_state *>(plan.handle)->bytes;
+  return bc_success;
+}
+
+bc_error bc_amd_plan_launch(bc_amd_plan plan, bc_stream stream) {
+  plan_state *p = static_cast<plan_state *>(plan.handle);
+  if (!p)
+    return bc_error_invalid_value;
+  BC_TRY_BC(hipGraphLaunch(p->exec, to_stream(stream)));
+  return as_bc(hipEventRecord(p->last, to_stream(stream)));
+}
+
+bc_error bc_amd_plan_destroy(bc_amd_plan plan) {
+  if (!plan.handle)
+    return bc_error_invalid_value;
+  plan_free(static_cast<plan_state *>(plan.handle));
+  return bc_success;
+}
+
+} // extern "C"

@@ -48,6 +48,44 @@ template <class T> hipError_t pool_alloc(T **ptr, size_t count, hipMemPool_t poo
 
 inline hipError_t pool_free(void *ptr, hipStream_t stream) { return ptr ? hipFreeAsync(ptr, stream) : hipSuccess; }
 
+// Scratch memory of one operation.  Pool mode takes stream-ordered
+// allocations from the caller's pool (plain C-ABI calls) and records the total
+// requested; bump mode carves a preallocated workspace (plans, hipGraph
+// capture: no allocation inside the captured region) and fails with
+// hipErrorOutOfMemory instead of overrunning it.
+class arena {
+public:
+  static arena pool_mode(hipMemPool_t pool, hipStream_t stream) { return arena(pool, stream, nullptr, 0, false); }
+  static arena bump_mode(void *base, size_t size, hipStream_t stream) { return arena(nullptr, stream, static_cast<char *>(base), size, true); }
+
+  template <class T> hipError_t get(T **ptr, size_t count) {
+    const size_t bytes = ((count * sizeof(T) > 0 ? count * sizeof(T) : 16) + 255) & ~size_t(255);
+    requested_ += bytes;
+    if (bump_) {
+      if (used_ + bytes > size_)
+        return hipErrorOutOfMemory;
+      *ptr = reinterpret_cast<T *>(base_ + used_);
+      used_ += bytes;
+      return hipSuccess;
+    }
+    return pool_alloc(ptr, bytes / sizeof(T) + 1, pool_, stream_);
+  }
+
+  hipError_t put(void *ptr) { return bump_ ? hipSuccess : pool_free(ptr, stream_); }
+
+  hipStream_t stream() const { return stream_; }
+  size_t requested() const { return requested_; }
+
+private:
+  arena(hipMemPool_t pool, hipStream_t stream, char *base, size_t size, bool bump)
+      : pool_(pool), stream_(stream), base_(base), size_(size), used_(0), requested_(0), bump_(bump) {}
+  hipMemPool_t pool_;
+  hipStream_t stream_;
+  char *base_;
+  size_t size_, used_, requested_;
+  bool bump_;
+};
+
 enum class mem_kind { device, host_pinned, host_pageable };
 
 // Classify a caller pointer as the reference does with cudaPointerGetAttributes

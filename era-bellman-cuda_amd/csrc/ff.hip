@@ -318,7 +318,8 @@ bc_error ff_x_mul_y(const void *x, const void *y, void *result, unsigned count, 
 
 // Inclusive prefix product (src/ff.cu:104-118).
 bc_error ff_grand_product(ff_grand_product_configuration cfg) {
-  return as_bc(prefix_product(static_cast<const fe *>(cfg.inputs), static_cast<fe *>(cfg.outputs), cfg.count, to_pool(cfg.mem_pool), to_stream(cfg.stream)));
+  arena ar = arena::pool_mode(to_pool(cfg.mem_pool), to_stream(cfg.stream));
+  return as_bc(prefix_product(static_cast<const fe *>(cfg.inputs), static_cast<fe *>(cfg.outputs), cfg.count, ar));
 }
 
 // outputs[i] = inputs[i] * base^i via a prefix product of [1, base, base, ...] (src/ff.cu:136-163).
@@ -337,7 +338,8 @@ bc_error ff_multiply_by_powers(ff_multiply_by_powers_configuration cfg) {
   BC_TRY_BC(ff_set_value_one(powers, 1, cfg.stream));
   if (n > 1) {
     BC_TRY_BC(ff_set_value(powers + 1, cfg.base, n - 1, cfg.stream));
-    BC_TRY_BC(prefix_product(powers + 1, powers + 1, n - 1, pool, s));
+    arena ar = arena::pool_mode(pool, s);
+    BC_TRY_BC(prefix_product(powers + 1, powers + 1, n - 1, ar));
   }
   BC_TRY_BC(ff_x_mul_y(cfg.inputs, powers, cfg.outputs, n, cfg.stream));
   if (!use_outputs)
@@ -374,7 +376,8 @@ bc_error ff_poly_evaluate(ff_poly_evaluate_configuration cfg) {
     ff_multiply_by_powers_configuration mp{cfg.mem_pool, cfg.stream, cfg.values, cfg.point, temp, n};
     BC_TRY_BC(ff_multiply_by_powers(mp));
   }
-  BC_TRY_BC(field_sum(temp, sum, n, pool, s));
+  arena ar = arena::pool_mode(pool, s);
+  BC_TRY_BC(field_sum(temp, sum, n, ar));
   BC_TRY_BC(hipMemcpyAsync(cfg.result, sum, sizeof(fe), hipMemcpyDefault, s));
   BC_TRY_BC(pool_free(temp, s));
   BC_TRY_BC(pool_free(sum, s));
@@ -446,8 +449,8 @@ bc_error ff_select(const void *source, void *destination, const unsigned *indexe
 }
 
 bc_error ff_sort_u32(ff_sort_u32_configuration cfg) {
-  return as_bc(radix_sort_keys(static_cast<const uint32_t *>(cfg.values), static_cast<uint32_t *>(cfg.sorted_values), cfg.count, 0, 32, to_pool(cfg.mem_pool),
-                               to_stream(cfg.stream)));
+  arena ar = arena::pool_mode(to_pool(cfg.mem_pool), to_stream(cfg.stream));
+  return as_bc(radix_sort_keys(static_cast<const uint32_t *>(cfg.values), static_cast<uint32_t *>(cfg.sorted_values), cfg.count, 0, 32, ar));
 }
 
 bc_error pn_set_values_from_packed_bits(void *values, const void *packet_bits, unsigned count, bc_stream stream) {

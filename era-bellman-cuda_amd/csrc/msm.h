@@ -8,12 +8,22 @@
 namespace bc {
 
 constexpr unsigned MSM_DEFAULT_PIECE = 32;
+constexpr unsigned MSM_MAX_JOBS = 64;
 
 // Window width table of the reference (src/msm.cu:12-37).
 unsigned msm_window_bits(unsigned log_n);
 
-// Whole MSM on device buffers; results = 254 Jacobian points.
-hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n, unsigned c, unsigned piece, g1_jacobian *results, hipMemPool_t pool,
-                      hipStream_t stream);
+struct msm_job {
+  const g1_affine *bases; // device, at least 2^log_n affine Montgomery points
+  const fe *scalars;      // device, 2^log_n Montgomery Fr
+  g1_jacobian *results; // device, 254 points
+  unsigned log_n;
+  unsigned c;     // window bits, 3..24
+  unsigned piece; // most points one thread accumulates, >= 1
+};
+
+// Runs up to MSM_MAX_JOBS MSMs with one launch per phase for the whole batch;
+// temporaries come from `ar`, everything is ordered on ar.stream().
+hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar);
 
 } // namespace bc

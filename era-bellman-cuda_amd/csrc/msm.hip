@@ -1,4 +1,7 @@
-// Pippenger G1 MSM for gfx950.
+// Pippenger G1 MSM for gfx950, batched: every phase is one launch over all the
+// MSMs of a batch (job tables travel by value in the kernel arguments, so a
+// batch is a fixed launch sequence with no device-side tables to set up).  The
+// plain C-ABI call is a batch of one.
 //
 // Contract (src/bellman-cuda.h:477-506, src/msm.cu, src/msm_kernels.cu):
 //   scalars are Montgomery Fr; window width c(log n) and W = ceil(254/c) as
@@ -10,24 +13,25 @@
 //   253 (src/msm.cu:354-414, src/msm_kernels.cu:269-299).  The caller folds
 //   them as sum = 2*sum + result[i].
 //
-// Pipeline (all on the caller's stream, temporaries from the caller's pool):
+// Pipeline:
 //   1. digits:     one thread per scalar writes, for each window, the key
-//                  w*2^(c-1) + (digit - 1) (the global bucket id) and the value
-//                  sign | point index;
-//   2. sort:       one stable LSD radix sort of all W*n (key, value) pairs
-//                  (8-bit digits, per-tile LDS histograms, wave-ballot ranking:
-//                  no global atomics, so heavy buckets cost nothing extra),
-//                  then run detection gives each bucket's [start, end);
-//   3. accumulate: buckets are cut into pieces of at most PIECE points; one
-//                  thread accumulates one piece with XYZZ mixed additions, so
-//                  a heavy bucket (e.g. digit 1 of window 0 when many scalars
-//                  are 1) is spread over many threads;
+//                  (batch-global bucket id, or NO_BUCKET for a zero digit) and
+//                  the value sign | point index;
+//   2. sort:       one stable LSD radix sort of all (key, value) pairs of the
+//                  batch (8-bit digits, per-tile LDS histograms, wave-ballot
+//                  ranking: no global atomics, so heavy buckets cost nothing
+//                  extra), then run detection gives each bucket's [start, end);
+//   3. accumulate: buckets are cut into pieces of at most `piece` points; one
+//                  thread accumulates one piece with XYZZ mixed additions, so a
+//                  heavy bucket (e.g. digit 1 of window 0 when many scalars are
+//                  1) is spread over many threads;
 //   4. fix-up:     buckets of up to 8 pieces are summed by one thread; heavier
 //                  buckets are listed and combined by a log-depth tree;
 //   5. per-bit reduction: windows are split recursively into low/high halves
-//                  of their index bits until each holds one bit (the sums are
-//                  split across threads and folded pairwise);
-//   6. gather:     254 results converted to Jacobian coordinates.
+//                  of their index bits until each holds one bit (sums split
+//                  across threads, then folded pairwise);
+//   6. gather:     254 Jacobian results per job.
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -67,11 +71,51 @@ unsigned msm_window_bits(unsigned log_n) {
 
 namespace {
 
-constexpr uint32_t SKIP = 0xffffffffu;
 constexpr uint32_t SIGN = 0x80000000u;
+constexpr uint32_t NO_BUCKET = 0x7fffffffu; // every key bit below 31 set: sorts after all buckets
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
 constexpr unsigned SPLIT_TERMS = 16;
+constexpr unsigned SMALL_PIECES = 8;
+
+struct job_desc {
+  const g1_affine *bases;
+  const fe *scalars;
+  uint64_t entry_begin; // first (key, value) pair of the job
+  uint32_t n;           // scalars
+  uint32_t c;           // window bits
+  uint32_t W;           // windows
+  uint32_t scalar_begin;
+  uint32_t bucket_begin; // first batch-global bucket
+  uint32_t top_begin;    // first top-digit slot
+  uint32_t piece;        // most points per accumulating thread
+};
+
+struct job_table {
+  uint32_t count;
+  job_desc job[MSM_MAX_JOBS];
+};
+static_assert(sizeof(job_table) <= 4096, "kernel argument budget");
+
+// Index of the last job whose `begin` field is <= v (fields ascend with the job index).
+template <class Begin> __device__ __forceinline__ unsigned find_job(const job_table &t, uint32_t v, Begin begin) {
+  unsigned lo = 0, hi = t.count;
+  while (hi - lo > 1) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (begin(t.job[mid]) <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+struct by_scalar {
+  __device__ uint32_t operator()(const job_desc &d) const { return d.scalar_begin; }
+};
+struct by_bucket {
+  __device__ uint32_t operator()(const job_desc &d) const { return d.bucket_begin; }
+};
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -86,25 +130,23 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-// 1. signed digits.  Entry (w, i) gets key = w * 2^(c-1) + (d - 1), i.e. the
-// global bucket id, or NO_BUCKET (sorts after every bucket) for d = 0, and
-// value = sign << 31 | i.
-constexpr uint32_t NO_BUCKET = 0x7fffffffu;
-
-__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict__ scalars, unsigned n, unsigned c, unsigned W, uint32_t *__restrict__ keys,
-                                                           uint32_t *__restrict__ values) {
+// 1. signed digits.  Entry (w, i) of a job gets key = bucket_begin + w*2^(c-1) + d - 1
+// (NO_BUCKET for d = 0) and value = sign << 31 | i.
+__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint32_t total, uint32_t *__restrict__ keys, uint32_t *__restrict__ values) {
   using F = FrParams;
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n)
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total)
     return;
-  const fe pos = from_mont<F>(load_fe(scalars + i));
+  const job_desc &jd = jobs.job[find_job(jobs, t, by_scalar())];
+  const uint32_t i = t - jd.scalar_begin, n = jd.n, c = jd.c;
+  const fe pos = from_mont<F>(load_fe(jd.scalars + i));
   fe neg;
-  sub_limbs(neg, fe_const<F>(F::m), pos); // r - s (r for s = 0), as neg<1>
+  sub_limbs(neg, fe_const<F>(F::m), pos); // r - s (r for s = 0)
   const uint32_t global_sign = lt(pos, neg) ? 0u : SIGN;
   const fe k = global_sign ? neg : pos;
   const uint32_t top = 1u << (c - 1);
   uint32_t borrow = 0;
-  for (unsigned w = 0; w < W; w++) {
+  for (unsigned w = 0; w < jd.W; w++) {
     uint32_t d = extract_bits(k, w * c, c) + borrow;
     uint32_t sign = global_sign;
     borrow = 0;
@@ -113,8 +155,8 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const fe *__restrict_
       borrow = 1;
       sign ^= SIGN;
     }
-    const size_t e = (size_t)w * n + i;
-    keys[e] = d ? w * top + (d - 1) : NO_BUCKET;
+    const uint64_t e = jd.entry_begin + (uint64_t)w * n + i;
+    keys[e] = d ? jd.bucket_begin + w * top + (d - 1) : NO_BUCKET;
     values[e] = sign | i;
   }
 }
@@ -135,10 +177,11 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const uint32_t *
 }
 
 // counts[b] = end[b] - start[b] (in place over `end`), pieces of at most `piece` points
-__global__ void piece_count_kernel(const uint32_t *start, uint32_t *counts, uint32_t *pieces, unsigned nb, unsigned piece) {
+__global__ void piece_count_kernel(const job_table jobs, const uint32_t *start, uint32_t *counts, uint32_t *pieces, unsigned nb) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb)
     return;
+  const uint32_t piece = jobs.job[find_job(jobs, b, by_bucket())].piece;
   const uint32_t cnt = counts[b] - start[b];
   counts[b] = cnt;
   pieces[b] = (cnt + piece - 1) / piece;
@@ -156,14 +199,17 @@ __global__ void piece_map_kernel(const uint32_t *piece_count, const uint32_t *pi
 }
 
 // 3. one thread accumulates one piece (<= piece points of one bucket)
-__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
                                                                const uint32_t *__restrict__ counts, const uint32_t *__restrict__ piece_offset,
                                                                const uint32_t *__restrict__ piece_bucket, const uint32_t *__restrict__ total,
-                                                               const g1_affine *__restrict__ bases, unsigned piece, g1_xyzz *__restrict__ partials) {
+                                                               g1_xyzz *__restrict__ partials) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= *total)
     return;
   const uint32_t b = piece_bucket[p];
+  const job_desc &jd = jobs.job[find_job(jobs, b, by_bucket())];
+  const g1_affine *__restrict__ bases = jd.bases;
+  const uint32_t piece = jd.piece;
   const uint32_t q = p - piece_offset[b];
   const uint32_t begin = offsets[b] + q * piece;
   const uint32_t stop = offsets[b] + counts[b];
@@ -181,7 +227,6 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const uint32_t *_
 
 // 4a. buckets of 2..SMALL_PIECES pieces are summed serially into piece 0;
 // larger ("heavy") buckets are listed for the tree below.
-constexpr unsigned SMALL_PIECES = 8;
 __global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count, unsigned nb,
                                                                 uint32_t *heavy, uint32_t *heavy_count) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,8 +278,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void heavy_tree_kernel(g1_xyzz *partials
                                                                const uint32_t *piece_count, unsigned stride) {
   const uint32_t total = *heavy_total, count = *heavy_count;
   for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
-    // heavy bucket holding flattened piece h: last j with heavy_offset[j] <= h
-    uint32_t lo = 0, hi = count;
+    uint32_t lo = 0, hi = count; // heavy bucket holding flattened piece h
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
       if (heavy_offset[mid] <= h)
@@ -251,229 +295,346 @@ __global__ __launch_bounds__(ACC_BLOCK) void heavy_tree_kernel(g1_xyzz *partials
   }
 }
 
-// level-0 bucket array A[w][k], k in [0, 2^(c-1)); tops[w] = bucket of the top digit
-__global__ void bucket_gather_kernel(const g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count, unsigned W, unsigned c,
-                                     unsigned top_signed, g1_xyzz *A, g1_xyzz *tops) {
-  const unsigned B = 1u << (c - 1);
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= W * B)
+// Level-0 windows: A[bucket_begin + w*2^(c-1) + k] for k in [0, 2^(c-1)) holds
+// bucket k of window w (k = 0 and the top digit hold infinity); tops[top_begin + w]
+// holds the bucket of the window's top digit (2^(c-1), or 2^top_signed in the last window).
+__global__ void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count,
+                                     g1_xyzz *A, g1_xyzz *tops) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb)
     return;
-  const unsigned w = t / B, k = t % B;
-  const unsigned top_digit = w == W - 1 ? (1u << top_signed) : B;
-  auto bucket_value = [&](unsigned d) {
-    const unsigned b = w * B + d - 1;
+  const job_desc &jd = jobs.job[find_job(jobs, t, by_bucket())];
+  const uint32_t B = 1u << (jd.c - 1);
+  const uint32_t local = t - jd.bucket_begin, w = local / B, k = local % B;
+  const uint32_t top_signed = jd.c - (jd.W * jd.c - 254) - 1;
+  const uint32_t top_digit = w == jd.W - 1 ? (1u << top_signed) : B;
+  auto bucket_value = [&](uint32_t d) {
+    const uint32_t b = jd.bucket_begin + w * B + d - 1;
     return piece_count[b] ? load_xyzz(partials + piece_offset[b]) : xyzz_infinity();
   };
   if (k == 0) {
     store_xyzz(A + t, xyzz_infinity());
     if (top_digit == B)
-      store_xyzz(tops + w, bucket_value(B));
+      store_xyzz(tops + jd.top_begin + w, bucket_value(B));
     return;
   }
   if (k == top_digit) {
     store_xyzz(A + t, xyzz_infinity());
-    store_xyzz(tops + w, bucket_value(k));
+    store_xyzz(tops + jd.top_begin + w, bucket_value(k));
     return;
   }
   store_xyzz(A + t, bucket_value(k));
 }
 
-// 5. split: windows of width s (2^s buckets) -> 2X windows of width t = ceil(s/2);
+// 5. one split level for every job whose windows are still wider than one bit.
+// Segment: X windows of width s at A -> 2X windows of width t = ceil(s/2) at out;
 // lo[u] = sum_v A[v*2^t + u] (2^hs terms), hi[v] = sum_u A[v*2^t + u] (2^t terms).
-// Output part p holds the terms == p (mod D_lo) of lo and (mod D_hi) of hi, with
-// D_hi = D_lo * 2^(t-hs) so every thread sums the same number of terms.
-__global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const g1_xyzz *A, unsigned X, unsigned s, unsigned D_lo, unsigned D_hi, g1_xyzz *out) {
-  const unsigned t = (s + 1) >> 1, hs = s - t;
-  const unsigned per_part = (2 * X) << t;
-  const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= D_hi * per_part)
-    return;
-  const unsigned part = id / per_part, rem = id % per_part;
-  const unsigned xo = rem >> t, j = rem & ((1u << t) - 1);
-  const unsigned x = xo >> 1, half = xo & 1;
-  const g1_xyzz *src = A + ((size_t)x << s);
-  g1_xyzz acc = xyzz_infinity();
-  if (half == 0) {
-    if (part < D_lo)
-      for (unsigned v = part; v < (1u << hs); v += D_lo)
-        acc = xyzz_add(acc, load_xyzz(src + ((v << t) | j)));
-  } else if (j < (1u << hs)) {
-    for (unsigned u = part; u < (1u << t); u += D_hi)
-      acc = xyzz_add(acc, load_xyzz(src + ((j << t) | u)));
-  }
-  store_xyzz(out + id, acc);
-}
-
-// out[i] += out[i + half_size] for i < half_size
-__global__ __launch_bounds__(ACC_BLOCK) void fold_parts_kernel(g1_xyzz *out, unsigned half_size) {
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= half_size)
-    return;
-  store_xyzz(out + i, xyzz_add(load_xyzz(out + i), load_xyzz(out + i + half_size)));
-}
-
-// 6. results
-struct result_map {
-  int32_t window[254]; // final width-1 window holding bit i, or -1 for top-digit bits
+// Output part p holds the terms == p (mod D_lo) of lo and (mod D) of hi, with
+// D = D_lo * 2^(t-hs), so every thread sums about the same number of terms.
+struct split_seg {
+  const g1_xyzz *A;
+  g1_xyzz *out;
+  uint32_t X, s, D_lo, D;
+  uint32_t thread_begin;
 };
 
-__global__ void result_kernel(const g1_xyzz *A_final, const g1_xyzz *tops, result_map map, unsigned c, g1_jacobian *out) {
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= 254)
-    return;
-  const int32_t x = map.window[i];
-  const g1_xyzz v = x < 0 ? load_xyzz(tops + i / c) : load_xyzz(A_final + 2 * (size_t)x + 1);
-  const g1_jacobian j = xyzz_to_jacobian(v);
-  store_fe(&out[i].x, j.x);
-  store_fe(&out[i].y, j.y);
-  store_fe(&out[i].z, j.z);
-}
+struct split_table {
+  uint32_t count;
+  split_seg seg[MSM_MAX_JOBS];
+};
 
-// Maps bits [b0, b0 + real) of a padded width-s window x onto final windows.
-void assign_bits(std::vector<int32_t> &bit_window, int x, unsigned b0, unsigned real, unsigned s) {
-  if (real == 0)
-    return;
-  if (s == 1) {
-    bit_window[b0] = x;
-    return;
+template <class Table> __device__ __forceinline__ unsigned find_seg(const Table &t, uint32_t v) {
+  unsigned lo = 0, hi = t.count;
+  while (hi - lo > 1) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (t.seg[mid].thread_begin <= v)
+      lo = mid;
+    else
+      hi = mid;
   }
-  const unsigned t = (s + 1) / 2;
-  assign_bits(bit_window, 2 * x, b0, real < t ? real : t, t);
-  if (real > t)
-    assign_bits(bit_window, 2 * x + 1, b0 + t, real - t, t);
+  return lo;
 }
 
-template <class T> hipError_t alloc(T **p, size_t n, hipMemPool_t pool, hipStream_t s) { return pool_alloc(p, n, pool, s); }
+__global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs, uint32_t total) {
+  const uint32_t id_all = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id_all >= total)
+    return;
+  const split_seg &sg = segs.seg[find_seg(segs, id_all)];
+  const uint32_t id = id_all - sg.thread_begin;
+  const unsigned s = sg.s, t = (s + 1) >> 1, hs = s - t;
+  const uint32_t per_part = (2 * sg.X) << t;
+  const uint32_t part = id / per_part, rem = id % per_part;
+  const uint32_t xo = rem >> t, j = rem & ((1u << t) - 1);
+  const uint32_t x = xo >> 1, half = xo & 1;
+  const g1_xyzz *src = sg.A + ((size_t)x << s);
+  g1_xyzz acc = xyzz_infinity();
+  if (half == 0) {
+    if (part < sg.D_lo)
+      for (uint32_t v = part; v < (1u << hs); v += sg.D_lo)
+        acc = xyzz_add(acc, load_xyzz(src + ((v << t) | j)));
+  } else if (j < (1u << hs)) {
+    for (uint32_t u = part; u < (1u << t); u += sg.D)
+      acc = xyzz_add(acc, load_xyzz(src + ((j << t) | u)));
+  }
+  store_xyzz(sg.out + id, acc);
+}
+
+// out[i] += out[i + half] for i < half, per segment
+struct fold_seg {
+  g1_xyzz *out;
+  uint32_t half;
+  uint32_t thread_begin;
+};
+
+struct fold_table {
+  uint32_t count;
+  fold_seg seg[MSM_MAX_JOBS];
+};
+
+__global__ __launch_bounds__(ACC_BLOCK) void fold_kernel(const fold_table segs, uint32_t total) {
+  const uint32_t id_all = blockIdx.x * blockDim.x + threadIdx.x;
+  if (id_all >= total)
+    return;
+  const fold_seg &sg = segs.seg[find_seg(segs, id_all)];
+  const uint32_t i = id_all - sg.thread_begin;
+  store_xyzz(sg.out + i, xyzz_add(load_xyzz(sg.out + i), load_xyzz(sg.out + i + sg.half)));
+}
+
+// 6. bit i = w*c + j of a job is its window's top bucket (j == c-1 or i == 253),
+// else bucket 1 of the one-bit window reached by following the splits: a width-s
+// window x holds bits [0, t) in child 2x and [t, s) in child 2x+1, t = ceil(s/2).
+struct result_desc {
+  const g1_xyzz *A_final;
+  const g1_xyzz *tops;
+  g1_jacobian *out;
+  uint32_t c;
+};
+
+struct result_table {
+  uint32_t count;
+  result_desc job[MSM_MAX_JOBS];
+};
+
+__global__ void result_kernel(const result_table jobs) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 254 * jobs.count)
+    return;
+  const result_desc &rd = jobs.job[t / 254];
+  const uint32_t i = t % 254, c = rd.c;
+  const uint32_t w = i / c;
+  uint32_t j = i % c;
+  g1_xyzz v;
+  if (j == c - 1 || i == 253) {
+    v = load_xyzz(rd.tops + w);
+  } else {
+    uint32_t x = w, s = c - 1;
+    while (s > 1) {
+      const uint32_t half = (s + 1) >> 1;
+      if (j < half) {
+        x = 2 * x;
+      } else {
+        x = 2 * x + 1;
+        j -= half;
+      }
+      s = half;
+    }
+    v = load_xyzz(rd.A_final + 2 * (size_t)x + 1);
+  }
+  const g1_jacobian r = xyzz_to_jacobian(v);
+  store_fe(&rd.out[i].x, r.x);
+  store_fe(&rd.out[i].y, r.y);
+  store_fe(&rd.out[i].z, r.z);
+}
 
 } // namespace
 
-// Full pipeline on device data: bases (2^log_n affine), scalars (device, Montgomery Fr),
-// results (device, 254 Jacobian).
-hipError_t msm_device(const g1_affine *bases, const fe *scalars, unsigned log_n, unsigned c, unsigned piece, g1_jacobian *results, hipMemPool_t pool,
-                      hipStream_t s) {
-  if (c < 3 || c > 22 || piece == 0)
+hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
+  hipStream_t s = ar.stream();
+  if (count == 0)
+    return hipSuccess;
+  if (count > MSM_MAX_JOBS)
     return hipErrorInvalidValue;
-  const unsigned n = 1u << log_n;
-  const unsigned W = (254 - 1) / c + 1;
-  const unsigned B = 1u << (c - 1);
-  const unsigned nb = W * B;
-  const unsigned top_signed = c - (W * c - 254) - 1;
-  const size_t entries = (size_t)W * n;
-  const size_t max_pieces = entries / piece + nb + 1;
+  job_table jt{};
+  jt.count = count;
+  uint64_t entries = 0, nb64 = 0;
+  uint32_t scalars_total = 0, tops_total = 0, max_np = 0;
+  uint64_t max_pieces = 1;
+  for (unsigned j = 0; j < count; j++) {
+    const unsigned c = jobs[j].c;
+    if (c < 3 || c > 24 || jobs[j].log_n > 28 || jobs[j].piece == 0 || !jobs[j].bases || !jobs[j].scalars || !jobs[j].results)
+      return hipErrorInvalidValue;
+    job_desc &d = jt.job[j];
+    d.bases = jobs[j].bases;
+    d.scalars = jobs[j].scalars;
+    d.n = 1u << jobs[j].log_n;
+    d.c = c;
+    d.W = (254 - 1) / c + 1;
+    d.scalar_begin = scalars_total;
+    d.entry_begin = entries;
+    d.bucket_begin = (uint32_t)nb64;
+    d.top_begin = tops_total;
+    d.piece = jobs[j].piece;
+    if ((uint64_t)scalars_total + d.n > 0xffffffffull)
+      return hipErrorInvalidValue;
+    scalars_total += d.n;
+    entries += (uint64_t)d.W * d.n;
+    nb64 += (uint64_t)d.W << (c - 1);
+    tops_total += d.W;
+    max_np = std::max(max_np, (d.n + d.piece - 1) / d.piece);
+    max_pieces += ((uint64_t)d.W * d.n + d.piece - 1) / d.piece + ((uint64_t)d.W << (c - 1)); // sum of ceil(count/piece) over buckets
+  }
+  if (entries >= (1ull << 31) || nb64 >= NO_BUCKET)
+    return hipErrorInvalidValue;
+  const uint32_t nb = (uint32_t)nb64;
 
-  uint32_t *counts = nullptr, *offsets = nullptr, *sorted = nullptr;
-  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *total = nullptr;
-  g1_xyzz *partials = nullptr;
-  BC_TRY(alloc(&counts, nb, pool, s));
-  BC_TRY(alloc(&offsets, nb, pool, s));
-  BC_TRY(alloc(&sorted, entries, pool, s));
-  BC_TRY(alloc(&piece_count, nb, pool, s));
-  BC_TRY(alloc(&piece_offset, nb, pool, s));
-  BC_TRY(alloc(&piece_bucket, max_pieces, pool, s));
-  BC_TRY(alloc(&total, 1, pool, s));
-  BC_TRY(alloc(&partials, max_pieces, pool, s));
-
+  // 1-2. digits, sort, bucket runs
+  uint32_t *offsets = nullptr, *counts = nullptr, *sorted = nullptr;
+  BC_TRY(ar.get(&offsets, nb));
+  BC_TRY(ar.get(&counts, nb));
+  BC_TRY(ar.get(&sorted, entries));
   {
     uint32_t *keys = nullptr, *vals = nullptr, *sorted_keys = nullptr;
-    BC_TRY(alloc(&keys, entries, pool, s));
-    BC_TRY(alloc(&vals, entries, pool, s));
-    BC_TRY(alloc(&sorted_keys, entries, pool, s));
-    digits_kernel<<<grid_for(n, MSM_BLOCK), MSM_BLOCK, 0, s>>>(scalars, n, c, W, keys, vals);
+    BC_TRY(ar.get(&keys, entries));
+    BC_TRY(ar.get(&vals, entries));
+    BC_TRY(ar.get(&sorted_keys, entries));
+    digits_kernel<<<grid_for(scalars_total, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, scalars_total, keys, vals);
     BC_LAUNCH_CHECK();
     unsigned key_bits = 0;
-    while ((1ull << key_bits) <= (uint64_t)W << (c - 1))
+    while ((1ull << key_bits) <= nb)
       key_bits++;
-    // NO_BUCKET has every bit set below bit 31: it sorts after all buckets on key_bits
-    BC_TRY(radix_sort_pairs(keys, sorted_keys, vals, sorted, entries, 0, key_bits, pool, s));
+    BC_TRY(radix_sort_pairs(keys, sorted_keys, vals, sorted, entries, 0, key_bits, ar));
     BC_TRY(fill_u32(offsets, nb, 0, s));
     BC_TRY(fill_u32(counts, nb, 0, s));
     bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(sorted_keys, entries, offsets, counts);
     BC_LAUNCH_CHECK();
-    BC_TRY(pool_free(keys, s));
-    BC_TRY(pool_free(vals, s));
-    BC_TRY(pool_free(sorted_keys, s));
+    BC_TRY(ar.put(keys));
+    BC_TRY(ar.put(vals));
+    BC_TRY(ar.put(sorted_keys));
   }
-  piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(offsets, counts, piece_count, nb, piece);
-  BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(piece_count, piece_offset, nb, pool, s));
-  piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, total);
-  BC_LAUNCH_CHECK();
-  accumulate_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(sorted, offsets, counts, piece_offset, piece_bucket, total, bases, piece, partials);
-  BC_LAUNCH_CHECK();
-  uint32_t *heavy = nullptr, *heavy_offset = nullptr, *heavy_meta = nullptr; // meta: count, total
-  BC_TRY(alloc(&heavy, nb, pool, s));
-  BC_TRY(alloc(&heavy_offset, nb, pool, s));
-  BC_TRY(alloc(&heavy_meta, 2, pool, s));
-  BC_TRY(fill_u32(heavy_meta, 2, 0, s));
-  small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, heavy_meta);
-  BC_LAUNCH_CHECK();
-  const unsigned max_np = (n + piece - 1) / piece;
-  if (max_np > SMALL_PIECES) {
-    heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, heavy_meta, piece_count, heavy_offset, heavy_meta + 1);
-    BC_LAUNCH_CHECK();
-    for (unsigned stride = 1; stride < max_np; stride <<= 1) {
-      heavy_tree_kernel<<<1024, ACC_BLOCK, 0, s>>>(partials, heavy, heavy_meta, heavy_offset, heavy_meta + 1, piece_offset, piece_count, stride);
-      BC_LAUNCH_CHECK();
-    }
-  }
-  BC_TRY(pool_free(heavy, s));
-  BC_TRY(pool_free(heavy_offset, s));
-  BC_TRY(pool_free(heavy_meta, s));
-  BC_TRY(pool_free(sorted, s));
 
-  // level-0 windows
+  // 3. pieces and accumulation; meta = {piece total, heavy count, heavy piece total}
+  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *meta = nullptr;
+  g1_xyzz *partials = nullptr;
+  BC_TRY(ar.get(&piece_count, nb));
+  BC_TRY(ar.get(&piece_offset, nb));
+  BC_TRY(ar.get(&piece_bucket, max_pieces));
+  BC_TRY(ar.get(&meta, 3));
+  BC_TRY(ar.get(&partials, max_pieces));
+  piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_count, nb);
+  BC_LAUNCH_CHECK();
+  BC_TRY(scan_u32_exclusive(piece_count, piece_offset, nb, ar));
+  piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, meta);
+  BC_LAUNCH_CHECK();
+  accumulate_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, offsets, counts, piece_offset, piece_bucket, meta, partials);
+  BC_LAUNCH_CHECK();
+
+  // 4. fix-up of multi-piece buckets
+  {
+    uint32_t *heavy = nullptr, *heavy_offset = nullptr;
+    BC_TRY(ar.get(&heavy, nb));
+    BC_TRY(ar.get(&heavy_offset, nb));
+    BC_TRY(fill_u32(meta + 1, 2, 0, s));
+    small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta + 1);
+    BC_LAUNCH_CHECK();
+    if (max_np > SMALL_PIECES) {
+      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta + 1, piece_count, heavy_offset, meta + 2);
+      BC_LAUNCH_CHECK();
+      for (unsigned stride = 1; stride < max_np; stride <<= 1) {
+        heavy_tree_kernel<<<1024, ACC_BLOCK, 0, s>>>(partials, heavy, meta + 1, heavy_offset, meta + 2, piece_offset, piece_count, stride);
+        BC_LAUNCH_CHECK();
+      }
+    }
+    BC_TRY(ar.put(heavy));
+    BC_TRY(ar.put(heavy_offset));
+  }
+
   g1_xyzz *A = nullptr, *tops = nullptr;
-  BC_TRY(alloc(&A, nb, pool, s));
-  BC_TRY(alloc(&tops, W, pool, s));
-  bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(partials, piece_offset, piece_count, W, c, top_signed, A, tops);
+  BC_TRY(ar.get(&A, nb));
+  BC_TRY(ar.get(&tops, tops_total));
+  bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops);
   BC_LAUNCH_CHECK();
-  BC_TRY(pool_free(partials, s));
-  BC_TRY(pool_free(piece_bucket, s));
-  BC_TRY(pool_free(piece_offset, s));
-  BC_TRY(pool_free(piece_count, s));
-  BC_TRY(pool_free(offsets, s));
-  BC_TRY(pool_free(counts, s));
-  BC_TRY(pool_free(total, s));
+  for (void *p : {(void *)partials, (void *)piece_bucket, (void *)piece_offset, (void *)piece_count, (void *)meta, (void *)offsets, (void *)counts,
+                  (void *)sorted})
+    BC_TRY(ar.put(p));
 
-  // 5. recursive splits until every window is one bit wide
-  unsigned width = c - 1, X = W;
-  while (width > 1) {
-    const unsigned t = (width + 1) / 2, hs = width - t;
-    const unsigned per_part = (2 * X) << t;
-    // about SPLIT_TERMS terms per thread
-    const unsigned D_lo = (1u << hs) > SPLIT_TERMS ? (1u << hs) / SPLIT_TERMS : 1u;
-    const unsigned D = D_lo << (t - hs);
+  // 5. split levels, all jobs together
+  std::vector<g1_xyzz *> cur(count), level_buffers;
+  std::vector<uint32_t> width(count), X(count);
+  for (unsigned j = 0; j < count; j++) {
+    cur[j] = A + jt.job[j].bucket_begin;
+    width[j] = jt.job[j].c - 1;
+    X[j] = jt.job[j].W;
+  }
+  for (;;) {
+    split_table st{};
+    std::vector<unsigned> active;
+    std::vector<uint32_t> per_part;
+    uint64_t threads = 0;
+    for (unsigned j = 0; j < count; j++) {
+      if (width[j] <= 1)
+        continue;
+      const unsigned t = (width[j] + 1) / 2, hs = width[j] - t;
+      const uint32_t D_lo = (1u << hs) > SPLIT_TERMS ? (1u << hs) / SPLIT_TERMS : 1u;
+      split_seg &sg = st.seg[st.count++];
+      sg.A = cur[j];
+      sg.X = X[j];
+      sg.s = width[j];
+      sg.D_lo = D_lo;
+      sg.D = D_lo << (t - hs);
+      sg.thread_begin = (uint32_t)threads;
+      per_part.push_back((2 * X[j]) << t);
+      threads += (uint64_t)sg.D * per_part.back();
+      active.push_back(j);
+    }
+    if (active.empty())
+      break;
+    if (threads >= (1ull << 32))
+      return hipErrorInvalidValue;
     g1_xyzz *next = nullptr;
-    BC_TRY(alloc(&next, (size_t)D * per_part, pool, s));
-    split_kernel<<<grid_for((size_t)D * per_part, ACC_BLOCK), ACC_BLOCK, 0, s>>>(A, X, width, D_lo, D, next);
+    BC_TRY(ar.get(&next, threads));
+    level_buffers.push_back(next);
+    for (unsigned k = 0; k < st.count; k++)
+      st.seg[k].out = next + st.seg[k].thread_begin;
+    split_kernel<<<grid_for(threads, ACC_BLOCK), ACC_BLOCK, 0, s>>>(st, (uint32_t)threads);
     BC_LAUNCH_CHECK();
-    for (unsigned d = D / 2; d >= 1; d /= 2) {
-      fold_parts_kernel<<<grid_for((size_t)d * per_part, ACC_BLOCK), ACC_BLOCK, 0, s>>>(next, d * per_part);
+    // fold round r halves every segment that still holds more than one part
+    for (unsigned r = 0;; r++) {
+      fold_table ft{};
+      uint32_t fthreads = 0;
+      for (unsigned k = 0; k < st.count; k++) {
+        const uint32_t d = st.seg[k].D >> (r + 1);
+        if (d == 0)
+          continue;
+        fold_seg &fs = ft.seg[ft.count++];
+        fs.out = st.seg[k].out;
+        fs.half = d * per_part[k];
+        fs.thread_begin = fthreads;
+        fthreads += fs.half;
+      }
+      if (ft.count == 0)
+        break;
+      fold_kernel<<<grid_for(fthreads, ACC_BLOCK), ACC_BLOCK, 0, s>>>(ft, fthreads);
       BC_LAUNCH_CHECK();
     }
-    BC_TRY(pool_free(A, s));
-    A = next;
-    X *= 2;
-    width = t;
+    for (unsigned k = 0; k < st.count; k++) {
+      const unsigned j = active[k];
+      cur[j] = st.seg[k].out;
+      X[j] *= 2;
+      width[j] = (width[j] + 1) / 2;
+    }
   }
 
-  // 6. per-bit results
-  std::vector<int32_t> bit_window(254, -1);
-  for (unsigned w = 0; w < W; w++) {
-    const unsigned real = w == W - 1 ? top_signed : c - 1;
-    std::vector<int32_t> local(c, -1);
-    assign_bits(local, (int)w, 0, real, c - 1);
-    for (unsigned j = 0; j < c - 1; j++)
-      if (w * c + j < 254 && j < real)
-        bit_window[w * c + j] = local[j];
-  }
-  result_map map{};
-  for (unsigned i = 0; i < 254; i++)
-    map.window[i] = bit_window[i];
-  result_kernel<<<1, 256, 0, s>>>(A, tops, map, c, results);
+  // 6. results
+  result_table rt{};
+  rt.count = count;
+  for (unsigned j = 0; j < count; j++)
+    rt.job[j] = {cur[j], tops + jt.job[j].top_begin, jobs[j].results, jt.job[j].c};
+  result_kernel<<<grid_for(254 * count, 256), 256, 0, s>>>(rt);
   BC_LAUNCH_CHECK();
-  BC_TRY(pool_free(A, s));
-  BC_TRY(pool_free(tops, s));
+  for (g1_xyzz *p : level_buffers)
+    BC_TRY(ar.put(p));
+  BC_TRY(ar.put(A));
+  BC_TRY(ar.put(tops));
   return hipSuccess;
 }
 
@@ -483,7 +644,7 @@ hipError_t msm_execute(const msm_configuration &cfg) {
   hipStream_t s = to_stream(cfg.stream);
   hipMemPool_t pool = to_pool(cfg.mem_pool);
   const unsigned L = cfg.log_scalars_count;
-  if (L > 31)
+  if (L > 28)
     return hipErrorInvalidValue;
   if (classify_pointer(cfg.bases) != mem_kind::device)
     return hipErrorInvalidValue;
@@ -507,7 +668,9 @@ hipError_t msm_execute(const msm_configuration &cfg) {
     BC_TRY(pool_alloc(&tmp_results, 254, pool, s));
     d_results = tmp_results;
   }
-  BC_TRY(msm_device(static_cast<const g1_affine *>(cfg.bases), d_scalars, L, msm_window_bits(L), MSM_DEFAULT_PIECE, d_results, pool, s));
+  const msm_job job{static_cast<const g1_affine *>(cfg.bases), d_scalars, d_results, L, msm_window_bits(L), MSM_DEFAULT_PIECE};
+  arena ar = arena::pool_mode(pool, s);
+  BC_TRY(msm_batch_device(&job, 1, ar));
   if (results_host) {
     BC_TRY(hipMemcpyAsync(cfg.results, tmp_results, sizeof(g1_jacobian) * 254, hipMemcpyDeviceToHost, s));
     if (cfg.d2h_copy_finished.handle)

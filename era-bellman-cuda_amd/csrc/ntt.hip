@@ -1,4 +1,7 @@
-// Fr NTT for gfx950.
+// Fr NTT for gfx950, batched: a batch of transforms of any sizes and
+// directions runs as one launch per pass index (per butterfly order), with
+// the per-transform descriptors passed by value in the kernel arguments.  The
+// plain C-ABI call is a batch of one.
 //
 // Semantics (src/ntt_kernels.cu:119-294, src/bellman-cuda.h:516-541):
 //   bit_reversed_inputs = false: natural order in, bit-reversed order out;
@@ -19,32 +22,35 @@
 // elements, strided by 2^(L-s_hi).  log n <= 24 takes at most two HBM passes.
 // Tiles are mapped XCD-major so that neighbouring strided columns run on the
 // same XCD and share its L2 lines.
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
 #include "common.h"
+#include "ntt.h"
 #include "tables.h"
+#include "tuning.h"
 
 namespace bc {
 
 namespace {
 
-constexpr unsigned NTT_MAX_LOG = 28;
 constexpr unsigned NTT_LOG_TILE = 12; // 4096 elements per tile
 constexpr unsigned NTT_BLOCK = 512;
 constexpr unsigned TWIST_FINE_BITS = 12;
 
 struct twiddle_state {
-  fe *fwd = nullptr; // TW[2^s + b] = omega_{2^(s+1)}^brev_s(b), index 0 unused
-  fe *inv = nullptr; // inverse powers, same layout
+  fe *fwd = nullptr;       // TW[2^s + b] = omega_{2^(s+1)}^brev_s(b), index 0 unused
+  fe *inv = nullptr;       // inverse powers, same layout
+  fe *inv_sizes = nullptr; // 2^-i, i <= NTT_MAX_LOG
   unsigned log_size = 0;
+  bool attributes = false;
   std::vector<fe *> retired;
 };
 
 constexpr int MAX_DEVICES = 16;
 twiddle_state g_tw[MAX_DEVICES];
 std::mutex g_tw_mutex;
-bool g_attr_set[MAX_DEVICES];
 
 int device_index() {
   int d = 0;
@@ -69,80 +75,67 @@ __global__ void twiddle_build_kernel(power_table w, fe *fwd, fe *inv, unsigned l
   store_fe(inv + m, table_pow(w, inverse_exponent(e)));
 }
 
-// Grow the twiddle tables to cover 2^log_n; older tables stay alive (in-flight
-// work may still read them) until ntt_tear_down.
-hipError_t ensure_twiddles(unsigned log_n, hipStream_t stream, const fe **fwd, const fe **inv) {
-  std::lock_guard<std::mutex> lock(g_tw_mutex);
-  twiddle_state &t = g_tw[device_index()];
-  if (t.log_size < log_n) {
-    const ff_state *ff = ff_current();
-    if (!ff)
-      return hipErrorNotInitialized;
-    const unsigned log_size = log_n < 12 ? 12 : log_n;
-    fe *f = nullptr, *i = nullptr;
-    BC_TRY(hipMalloc(&f, sizeof(fe) << log_size));
-    BC_TRY(hipMalloc(&i, sizeof(fe) << log_size));
-    twiddle_build_kernel<<<grid_for(1u << log_size, 256), 256, 0, stream>>>(ff->w, f, i, log_size);
-    BC_LAUNCH_CHECK();
-    // other streams may use the tables as soon as they are published
-    BC_TRY(hipStreamSynchronize(stream));
-    if (t.fwd) {
-      t.retired.push_back(t.fwd);
-      t.retired.push_back(t.inv);
-    }
-    t.fwd = f;
-    t.inv = i;
-    t.log_size = log_size;
-  }
-  *fwd = t.fwd;
-  *inv = t.inv;
-  return hipSuccess;
-}
+enum : uint8_t { POST_LAZY = 0, POST_REDUCE = 1, POST_SCALE = 2, POST_TABLE = 3 };
 
-enum : unsigned { POST_LAZY = 0, POST_REDUCE = 1, POST_SCALE = 2, POST_TABLE = 3 };
-
-struct pass_args {
+// One transform's share of a pass launch.
+struct pass_job {
   const fe *in;
   fe *out;
-  const fe *tw;
-  unsigned log_n, s_lo, s_hi;
-  unsigned log_t, log_c, log_g; // tile = 2^log_g groups x 2^log_t rows x 2^log_c columns
-  unsigned tiles;
-  unsigned xcd_remap; // tiles divisible by 8
-  bool pre_table;     // multiply element i by twist(natural index)
-  unsigned post;      // POST_*
-  fe scale;           // POST_SCALE constant
-  const fe *twist_fine;
-  const fe *twist_coarse;
+  const fe *aux; // POST_SCALE: n^-1; twists: fine table, coarse table 2^12 entries later
+  uint32_t tile_begin; // first block of the job (a multiple of 8)
+  uint8_t log_n, s_lo, s_hi;
+  uint8_t log_c, log_g; // tile = 2^log_g groups x 2^(s_hi-s_lo) rows x 2^log_c columns
+  uint8_t inverse, pre_table, post;
 };
 
-__device__ __forceinline__ fe twist_pow(const pass_args &a, unsigned i) {
-  const fe c = load_fe(a.twist_coarse + (i >> TWIST_FINE_BITS));
+struct pass_table {
+  uint32_t count;
+  const fe *tw_fwd;
+  const fe *tw_inv;
+  pass_job job[NTT_MAX_JOBS];
+};
+static_assert(sizeof(pass_table) <= 4096, "kernel argument budget");
+
+__device__ __forceinline__ fe twist_pow(const fe *fine, unsigned i) {
+  const fe c = load_fe(fine + (1u << TWIST_FINE_BITS) + (i >> TWIST_FINE_BITS));
   const unsigned f = i & ((1u << TWIST_FINE_BITS) - 1);
-  return f ? mul<FrParams>(load_fe(a.twist_fine + f), c) : c;
+  return f ? mul<FrParams>(load_fe(fine + f), c) : c;
 }
 
-template <bool GS>
-__global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) {
+template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_table pt) {
   using F = FrParams;
   extern __shared__ fe tile[];
-  unsigned tid_tile = blockIdx.x;
-  if (a.xcd_remap)
-    tid_tile = (blockIdx.x & 7) * (a.tiles >> 3) + (blockIdx.x >> 3);
-  const unsigned L = a.log_n;
-  const unsigned log_lo = L - a.s_hi;           // bits below the group
-  const unsigned lo_tiles_log = log_lo - a.log_c; // tiles along the low bits
-  const unsigned lo0 = (tid_tile & ((1u << lo_tiles_log) - 1)) << a.log_c;
-  const unsigned hi0 = (tid_tile >> lo_tiles_log) << a.log_g;
-  const unsigned elems = 1u << (a.log_g + a.log_t + a.log_c);
-  const unsigned cmask = (1u << a.log_c) - 1, tmask = (1u << a.log_t) - 1;
+  unsigned jl = 0, jh = pt.count; // job of this block
+  while (jh - jl > 1) {
+    const unsigned mid = (jl + jh) >> 1;
+    if (pt.job[mid].tile_begin <= blockIdx.x)
+      jl = mid;
+    else
+      jh = mid;
+  }
+  const pass_job &a = pt.job[jl];
+  const unsigned L = a.log_n, s_lo = a.s_lo, s_hi = a.s_hi, log_c = a.log_c, log_g = a.log_g;
+  const unsigned log_t = s_hi - s_lo;
+  const unsigned tiles = 1u << (L - log_t - log_c - log_g);
+  unsigned tid_tile = blockIdx.x - a.tile_begin;
+  if (tid_tile >= tiles)
+    return; // alignment padding
+  if ((tiles & 7) == 0)
+    tid_tile = (tid_tile & 7) * (tiles >> 3) + (tid_tile >> 3);
+  const fe *__restrict__ tw = a.inverse ? pt.tw_inv : pt.tw_fwd;
+  const unsigned log_lo = L - s_hi;             // bits below the group
+  const unsigned lo_tiles_log = log_lo - log_c; // tiles along the low bits
+  const unsigned lo0 = (tid_tile & ((1u << lo_tiles_log) - 1)) << log_c;
+  const unsigned hi0 = (tid_tile >> lo_tiles_log) << log_g;
+  const unsigned elems = 1u << (log_g + log_t + log_c);
+  const unsigned cmask = (1u << log_c) - 1, tmask = (1u << log_t) - 1;
 
   for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK) {
-    const unsigned c = e & cmask, t = (e >> a.log_c) & tmask, g = e >> (a.log_c + a.log_t);
-    const unsigned idx = ((hi0 + g) << (L - a.s_lo)) | (t << log_lo) | (lo0 + c);
+    const unsigned c = e & cmask, t = (e >> log_c) & tmask, g = e >> (log_c + log_t);
+    const unsigned idx = ((hi0 + g) << (L - s_lo)) | (t << log_lo) | (lo0 + c);
     fe v = load_fe(a.in + idx);
     if (a.pre_table)
-      v = mul_lazy<F>(v, twist_pow(a, GS ? brev_bits(idx, L) : idx));
+      v = mul_lazy<F>(v, twist_pow(a.aux, GS ? brev_bits(idx, L) : idx));
     tile[e] = v;
   }
   __syncthreads();
@@ -151,29 +144,29 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
   // count starts with one radix-2 step; the rest go two at a time: a thread
   // holds the 4 elements t0, t0+h/2, t0+h, t0+3h/2 (h = row distance of the
   // lower stage s) and applies stages s and s+1 in registers.
-  const unsigned nst = a.s_hi - a.s_lo;
+  const unsigned nst = log_t;
   unsigned done = 0;
   if (nst & 1) {
-    const unsigned s = GS ? a.s_hi - 1 : a.s_lo;
-    const unsigned log_h = a.s_hi - s - 1;
+    const unsigned s = GS ? s_hi - 1 : s_lo;
+    const unsigned log_h = s_hi - s - 1;
     const unsigned h = 1u << log_h;
     for (unsigned q = threadIdx.x; q < (elems >> 1); q += NTT_BLOCK) {
       const unsigned c = q & cmask;
-      const unsigned r = (q >> a.log_c) & (tmask >> 1);
-      const unsigned g = q >> (a.log_c + a.log_t - 1);
+      const unsigned r = (q >> log_c) & (tmask >> 1);
+      const unsigned g = q >> (log_c + log_t - 1);
       const unsigned t0 = ((r >> log_h) << (log_h + 1)) | (r & (h - 1));
-      const unsigned i0 = (((g << a.log_t) | t0) << a.log_c) | c;
-      const unsigned i1 = i0 + (h << a.log_c);
-      const unsigned b = ((hi0 + g) << (s - a.s_lo)) | (t0 >> (a.s_hi - s));
+      const unsigned i0 = (((g << log_t) | t0) << log_c) | c;
+      const unsigned i1 = i0 + (h << log_c);
+      const unsigned b = ((hi0 + g) << (s - s_lo)) | (t0 >> (s_hi - s));
       const fe E = tile[i0], O = tile[i1];
       if (!GS) {
-        const fe wo = s != 0 ? mul_lazy<F>(O, load_fe(a.tw + (1u << s) + b)) : O;
+        const fe wo = s != 0 ? mul_lazy<F>(O, load_fe(tw + (1u << s) + b)) : O;
         tile[i0] = add2<F>(E, wo);
         tile[i1] = sub2<F>(E, wo);
       } else {
         const fe d = sub2<F>(E, O);
         tile[i0] = add2<F>(E, O);
-        tile[i1] = s != 0 ? mul_lazy<F>(d, load_fe(a.tw + (1u << s) + b)) : d;
+        tile[i1] = s != 0 ? mul_lazy<F>(d, load_fe(tw + (1u << s) + b)) : d;
       }
     }
     __syncthreads();
@@ -181,22 +174,22 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
   }
   for (; done < nst; done += 2) {
     // the lower-numbered stage of the pair
-    const unsigned s = GS ? a.s_hi - 2 - done : a.s_lo + done;
-    const unsigned log_h = a.s_hi - s - 1; // >= 1
+    const unsigned s = GS ? s_hi - 2 - done : s_lo + done;
+    const unsigned log_h = s_hi - s - 1; // >= 1
     const unsigned h = 1u << log_h, hh = h >> 1;
     for (unsigned q = threadIdx.x; q < (elems >> 2); q += NTT_BLOCK) {
       const unsigned c = q & cmask;
-      const unsigned u = (q >> a.log_c) & (tmask >> 2);
-      const unsigned g = q >> (a.log_c + a.log_t - 2);
+      const unsigned u = (q >> log_c) & (tmask >> 2);
+      const unsigned g = q >> (log_c + log_t - 2);
       const unsigned t0 = ((u >> (log_h - 1)) << (log_h + 1)) | (u & (hh - 1));
-      const unsigned i0 = (((g << a.log_t) | t0) << a.log_c) | c;
-      const unsigned i1 = i0 + (hh << a.log_c), i2 = i0 + (h << a.log_c), i3 = i2 + (hh << a.log_c);
-      const unsigned b = ((hi0 + g) << (s - a.s_lo)) | (t0 >> (a.s_hi - s));
-      const fe *tw_s1 = a.tw + (2u << s) + 2 * b; // stage s+1 twiddles of blocks 2b, 2b+1
+      const unsigned i0 = (((g << log_t) | t0) << log_c) | c;
+      const unsigned i1 = i0 + (hh << log_c), i2 = i0 + (h << log_c), i3 = i2 + (hh << log_c);
+      const unsigned b = ((hi0 + g) << (s - s_lo)) | (t0 >> (s_hi - s));
+      const fe *tw_s1 = tw + (2u << s) + 2 * b; // stage s+1 twiddles of blocks 2b, 2b+1
       fe x0 = tile[i0], x1 = tile[i1], x2 = tile[i2], x3 = tile[i3];
       if (!GS) {
         if (s != 0) {
-          const fe wa = load_fe(a.tw + (1u << s) + b);
+          const fe wa = load_fe(tw + (1u << s) + b);
           x2 = mul_lazy<F>(x2, wa);
           x3 = mul_lazy<F>(x3, wa);
         }
@@ -213,7 +206,7 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
         const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub2<F>(x2, x3), load_fe(tw_s1 + 1));
         fe z2 = sub2<F>(y0, y2), z3 = sub2<F>(y1, y3);
         if (s != 0) {
-          const fe wa = load_fe(a.tw + (1u << s) + b);
+          const fe wa = load_fe(tw + (1u << s) + b);
           z2 = mul_lazy<F>(z2, wa);
           z3 = mul_lazy<F>(z3, wa);
         }
@@ -226,19 +219,21 @@ __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(const pass_args a) 
     __syncthreads();
   }
 
+  const unsigned post = a.post;
+  const fe scale = post == POST_SCALE ? load_fe(a.aux) : fe_zero();
   for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK) {
-    const unsigned c = e & cmask, t = (e >> a.log_c) & tmask, g = e >> (a.log_c + a.log_t);
-    const unsigned idx = ((hi0 + g) << (L - a.s_lo)) | (t << log_lo) | (lo0 + c);
+    const unsigned c = e & cmask, t = (e >> log_c) & tmask, g = e >> (log_c + log_t);
+    const unsigned idx = ((hi0 + g) << (L - s_lo)) | (t << log_lo) | (lo0 + c);
     fe v = tile[e];
-    switch (a.post) {
+    switch (post) {
     case POST_REDUCE:
       v = reduce<F>(v);
       break;
     case POST_SCALE:
-      v = mul<F>(v, a.scale);
+      v = mul<F>(v, scale);
       break;
     case POST_TABLE:
-      v = mul<F>(v, twist_pow(a, GS ? idx : brev_bits(idx, L)));
+      v = mul<F>(v, twist_pow(a.aux, GS ? idx : brev_bits(idx, L)));
       break;
     default:
       break;
@@ -262,9 +257,16 @@ struct pass_plan {
   unsigned s_lo, s_hi;
 };
 
+// Stage runs of the HBM passes: ceil(L/12) passes, balanced unless the tuning
+// table names the first pass of a two-pass size.
 std::vector<pass_plan> plan_passes(unsigned L) {
   std::vector<pass_plan> p;
   const unsigned passes = (L + NTT_LOG_TILE - 1) / NTT_LOG_TILE;
+  if (passes == 2) {
+    const unsigned first = ntt_first_pass(L);
+    if (first && first < L && first <= NTT_LOG_TILE && L - first <= NTT_LOG_TILE)
+      return {{0, first}, {first, L}};
+  }
   unsigned s = 0;
   for (unsigned i = 0; i < passes; i++) {
     const unsigned len = L / passes + (i < L % passes ? 1 : 0);
@@ -274,98 +276,170 @@ std::vector<pass_plan> plan_passes(unsigned L) {
   return p;
 }
 
-hipError_t set_kernel_attributes() {
-  const int d = device_index();
-  if (g_attr_set[d])
-    return hipSuccess;
-  const int bytes = (int)(sizeof(fe) << NTT_LOG_TILE);
-  BC_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<false>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  BC_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<true>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
-  g_attr_set[d] = true;
-  return hipSuccess;
+fe inverse_power_of_two(unsigned L) {
+  using F = FrParams;
+  const fe two_inv = bc::inverse<F>(add<F>(fe_one<F>(), fe_one<F>()));
+  fe r = fe_one<F>();
+  for (unsigned i = 0; i < L; i++)
+    r = mul<F>(r, two_inv);
+  return r;
 }
 
 } // namespace
 
-// Runs the whole transform on device buffers (in may equal out).
-hipError_t ntt_device(const fe *in, fe *out, unsigned L, bool bitrev_in, bool inverse, unsigned log_ext, unsigned coset, hipMemPool_t pool,
-                      hipStream_t stream) {
-  using F = FrParams;
-  if (L == 0 || L > NTT_MAX_LOG || L + log_ext > NTT_MAX_LOG)
-    return hipErrorInvalidValue;
-  if (log_ext && coset >= (1u << log_ext))
-    return hipErrorInvalidValue;
-  BC_TRY(set_kernel_attributes());
-  const fe *tw_f = nullptr, *tw_i = nullptr;
-  BC_TRY(ensure_twiddles(L, stream, &tw_f, &tw_i));
-
-  // twist table: forward coset -> alpha = rho; inverse -> n^-1 * rho^-i (or just n^-1)
-  fe n_inv = fe_one<F>();
-  {
-    fe two_inv = bc::inverse<F>(add<F>(fe_one<F>(), fe_one<F>()));
-    for (unsigned i = 0; i < L; i++)
-      n_inv = mul<F>(n_inv, two_inv);
+// Twiddles for 2^max_log_n and kernel attributes of the current device.  Growing
+// the tables synchronises `stream`; older tables stay alive until ntt_tear_down
+// because in-flight work may still read them.
+hipError_t ntt_prepare(unsigned max_log_n, hipStream_t stream, ntt_tables *out) {
+  std::lock_guard<std::mutex> lock(g_tw_mutex);
+  twiddle_state &t = g_tw[device_index()];
+  if (!t.attributes) {
+    const int bytes = (int)(sizeof(fe) << NTT_LOG_TILE);
+    BC_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<false>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    BC_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<true>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
+    t.attributes = true;
   }
-  fe *twist = nullptr;
-  const unsigned coarse_count = L > TWIST_FINE_BITS ? 1u << (L - TWIST_FINE_BITS) : 1u;
-  if (log_ext) {
+  if (!t.inv_sizes) {
+    std::vector<fe> sizes(NTT_MAX_LOG + 1);
+    for (unsigned i = 0; i <= NTT_MAX_LOG; i++)
+      sizes[i] = inverse_power_of_two(i);
+    fe *d = nullptr;
+    BC_TRY(hipMalloc(&d, sizeof(fe) * sizes.size()));
+    BC_TRY(hipMemcpy(d, sizes.data(), sizeof(fe) * sizes.size(), hipMemcpyHostToDevice));
+    t.inv_sizes = d;
+  }
+  if (t.log_size < max_log_n) {
+    const ff_state *ff = ff_current();
+    if (!ff)
+      return hipErrorNotInitialized;
+    const unsigned log_size = std::max(max_log_n, 12u);
+    fe *f = nullptr, *i = nullptr;
+    BC_TRY(hipMalloc(&f, sizeof(fe) << log_size));
+    BC_TRY(hipMalloc(&i, sizeof(fe) << log_size));
+    twiddle_build_kernel<<<grid_for(1u << log_size, 256), 256, 0, stream>>>(ff->w, f, i, log_size);
+    BC_LAUNCH_CHECK();
+    // other streams may use the tables as soon as they are published
+    BC_TRY(hipStreamSynchronize(stream));
+    if (t.fwd) {
+      t.retired.push_back(t.fwd);
+      t.retired.push_back(t.inv);
+    }
+    t.fwd = f;
+    t.inv = i;
+    t.log_size = log_size;
+  }
+  if (out)
+    *out = ntt_tables{t.fwd, t.inv, t.inv_sizes, t.log_size};
+  return hipSuccess;
+}
+
+hipError_t ntt_batch_device(const ntt_job *jobs, unsigned count, arena &ar) {
+  using F = FrParams;
+  hipStream_t stream = ar.stream();
+  if (count == 0)
+    return hipSuccess;
+  if (count > NTT_MAX_JOBS)
+    return hipErrorInvalidValue;
+  unsigned max_log = 0;
+  for (unsigned j = 0; j < count; j++) {
+    const ntt_job &q = jobs[j];
+    if (q.log_n == 0 || q.log_n > NTT_MAX_LOG || q.log_n + q.log_ext > NTT_MAX_LOG || !q.in || !q.out)
+      return hipErrorInvalidValue;
+    if (q.log_ext && q.coset >= (1u << q.log_ext))
+      return hipErrorInvalidValue;
+    max_log = std::max(max_log, q.log_n);
+  }
+  ntt_tables tt{};
+  BC_TRY(ntt_prepare(max_log, stream, &tt));
+
+  // per-job twist tables: forward coset -> rho^i; inverse -> n^-1 * rho^-i
+  std::vector<fe *> twist(count, nullptr);
+  for (unsigned j = 0; j < count; j++) {
+    const ntt_job &q = jobs[j];
+    if (!q.log_ext)
+      continue;
+    const unsigned L = q.log_n;
+    const unsigned coarse_count = L > TWIST_FINE_BITS ? 1u << (L - TWIST_FINE_BITS) : 1u;
     fe seven = fe_zero();
     seven.v[0] = F::generator;
     const fe g = to_mont<F>(seven);
-    const fe w_big = pow_u64<F>(fe_const<F>(F::omega), 1ull << (F::omega_log_order - L - log_ext));
-    fe rho = mul<F>(g, pow_u64<F>(w_big, coset));
-    if (inverse)
+    const fe w_big = pow_u64<F>(fe_const<F>(F::omega), 1ull << (F::omega_log_order - L - q.log_ext));
+    fe rho = mul<F>(g, pow_u64<F>(w_big, q.coset));
+    if (q.inverse)
       rho = bc::inverse<F>(rho);
     const fe rho_4096 = pow_u64<F>(rho, 1ull << TWIST_FINE_BITS);
-    BC_TRY(pool_alloc(&twist, (1u << TWIST_FINE_BITS) + coarse_count, pool, stream));
-    const unsigned threads = (1u << TWIST_FINE_BITS) + coarse_count;
-    twist_table_kernel<<<grid_for(threads, 256), 256, 0, stream>>>(twist, twist + (1u << TWIST_FINE_BITS), rho, rho_4096, inverse ? n_inv : fe_one<F>(),
-                                                                   coarse_count);
+    const unsigned entries = (1u << TWIST_FINE_BITS) + coarse_count;
+    BC_TRY(ar.get(&twist[j], entries));
+    twist_table_kernel<<<grid_for(entries, 256), 256, 0, stream>>>(twist[j], twist[j] + (1u << TWIST_FINE_BITS), rho, rho_4096,
+                                                                  q.inverse ? inverse_power_of_two(L) : fe_one<F>(), coarse_count);
     BC_LAUNCH_CHECK();
   }
 
-  const std::vector<pass_plan> plan = plan_passes(L);
-  const unsigned np = (unsigned)plan.size();
-  for (unsigned k = 0; k < np; k++) {
-    const pass_plan pp = bitrev_in ? plan[np - 1 - k] : plan[k];
-    pass_args a{};
-    a.in = k == 0 ? in : out;
-    a.out = out;
-    a.tw = inverse ? tw_i : tw_f;
-    a.log_n = L;
-    a.s_lo = pp.s_lo;
-    a.s_hi = pp.s_hi;
-    a.log_t = pp.s_hi - pp.s_lo;
-    const unsigned log_lo = L - pp.s_hi, log_hi = pp.s_lo;
-    const unsigned room = NTT_LOG_TILE - a.log_t;
-    a.log_c = log_lo < room ? log_lo : room;
-    a.log_g = a.log_c == log_lo ? ((room - a.log_c) < log_hi ? room - a.log_c : log_hi) : 0;
-    a.tiles = 1u << (L - a.log_t - a.log_c - a.log_g);
-    a.xcd_remap = (a.tiles % 8 == 0) ? 1u : 0u;
-    a.pre_table = k == 0 && log_ext && !inverse;
-    a.twist_fine = twist;
-    a.twist_coarse = twist ? twist + (1u << TWIST_FINE_BITS) : nullptr;
-    if (k == np - 1) {
-      if (!inverse)
-        a.post = POST_REDUCE;
-      else if (log_ext)
-        a.post = POST_TABLE;
-      else {
-        a.post = POST_SCALE;
-        a.scale = n_inv;
-      }
-    } else {
-      a.post = POST_LAZY;
-    }
-    const size_t lds = sizeof(fe) << (a.log_g + a.log_t + a.log_c);
-    if (bitrev_in)
-      ntt_pass_kernel<true><<<a.tiles, NTT_BLOCK, lds, stream>>>(a);
-    else
-      ntt_pass_kernel<false><<<a.tiles, NTT_BLOCK, lds, stream>>>(a);
-    BC_LAUNCH_CHECK();
+  std::vector<std::vector<pass_plan>> plans(count);
+  unsigned max_passes = 0;
+  for (unsigned j = 0; j < count; j++) {
+    plans[j] = plan_passes(jobs[j].log_n);
+    max_passes = std::max(max_passes, (unsigned)plans[j].size());
   }
-  if (twist)
-    BC_TRY(pool_free(twist, stream));
+  for (unsigned k = 0; k < max_passes; k++) {
+    for (int gs = 0; gs < 2; gs++) {
+      pass_table pt{};
+      pt.tw_fwd = tt.fwd;
+      pt.tw_inv = tt.inv;
+      uint32_t blocks = 0;
+      size_t lds = 0;
+      for (unsigned j = 0; j < count; j++) {
+        const ntt_job &q = jobs[j];
+        const unsigned np = (unsigned)plans[j].size();
+        if ((int)q.bitrev_in != gs || k >= np)
+          continue;
+        const pass_plan pp = q.bitrev_in ? plans[j][np - 1 - k] : plans[j][k];
+        const unsigned L = q.log_n;
+        const unsigned log_t = pp.s_hi - pp.s_lo;
+        const unsigned log_lo = L - pp.s_hi, log_hi = pp.s_lo;
+        const unsigned room = NTT_LOG_TILE - log_t;
+        const unsigned log_c = std::min(log_lo, room);
+        const unsigned log_g = log_c == log_lo ? std::min(room - log_c, log_hi) : 0;
+        pass_job &a = pt.job[pt.count++];
+        a.in = k == 0 ? q.in : q.out;
+        a.out = q.out;
+        a.log_n = (uint8_t)L;
+        a.s_lo = (uint8_t)pp.s_lo;
+        a.s_hi = (uint8_t)pp.s_hi;
+        a.log_c = (uint8_t)log_c;
+        a.log_g = (uint8_t)log_g;
+        a.inverse = q.inverse ? 1 : 0;
+        a.pre_table = (k == 0 && q.log_ext && !q.inverse) ? 1 : 0;
+        a.aux = twist[j];
+        if (k == np - 1) {
+          if (!q.inverse)
+            a.post = POST_REDUCE;
+          else if (q.log_ext)
+            a.post = POST_TABLE;
+          else {
+            a.post = POST_SCALE;
+            a.aux = tt.inv_sizes + L;
+          }
+        } else {
+          a.post = POST_LAZY;
+        }
+        a.tile_begin = blocks;
+        const uint32_t tiles = 1u << (L - log_t - log_c - log_g);
+        blocks += (tiles + 7) & ~7u;
+        lds = std::max(lds, sizeof(fe) << (log_g + log_t + log_c));
+      }
+      if (pt.count == 0)
+        continue;
+      if (gs)
+        ntt_pass_kernel<true><<<blocks, NTT_BLOCK, lds, stream>>>(pt);
+      else
+        ntt_pass_kernel<false><<<blocks, NTT_BLOCK, lds, stream>>>(pt);
+      BC_LAUNCH_CHECK();
+    }
+  }
+  for (fe *p : twist)
+    if (p)
+      BC_TRY(ar.put(p));
   return hipSuccess;
 }
 
@@ -379,6 +453,8 @@ hipError_t ntt_tear_down_impl() {
     BC_TRY(hipFree(t.fwd));
     BC_TRY(hipFree(t.inv));
   }
+  if (t.inv_sizes)
+    BC_TRY(hipFree(t.inv_sizes));
   t = twiddle_state{};
   return hipSuccess;
 }
@@ -386,9 +462,7 @@ hipError_t ntt_tear_down_impl() {
 hipError_t ntt_set_up_impl() {
   if (!ff_current())
     return hipErrorNotInitialized;
-  BC_TRY(set_kernel_attributes());
-  const fe *f, *i;
-  BC_TRY(ensure_twiddles(20, nullptr, &f, &i));
+  BC_TRY(ntt_prepare(20, nullptr, nullptr));
   return hipDeviceSynchronize();
 }
 
@@ -423,7 +497,9 @@ hipError_t ntt_execute(const ntt_configuration &c) {
       d_out = temp;
     }
   }
-  BC_TRY(ntt_device(d_in, d_out, L, c.bit_reversed_inputs, c.inverse, c.log_extension_degree, c.coset_index, pool, s));
+  const ntt_job job{d_in, d_out, L, c.bit_reversed_inputs, c.inverse, c.log_extension_degree, c.coset_index};
+  arena ar = arena::pool_mode(pool, s);
+  BC_TRY(ntt_batch_device(&job, 1, ar));
   if (out_host) {
     BC_TRY(hipMemcpyAsync(c.outputs, d_out, bytes, hipMemcpyDeviceToHost, s));
     if (c.d2h_copy_finished.handle)

@@ -8,6 +8,7 @@
 // group (cyclically), except the first group (smallest key), which keeps the
 // identity.  The output is column-major, canonical Montgomery Fr.
 #include "common.h"
+#include "pn.h"
 #include "scan.h"
 #include "sort.h"
 #include "tables.h"
@@ -66,43 +67,41 @@ __global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint3
 
 } // namespace
 
-hipError_t pn_generate(const generate_permutation_polynomials_configuration &cfg) {
+// Whole permutation on device buffers; temporaries from `ar`.
+hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *target, unsigned cols, unsigned L, arena &ar) {
   const ff_state *ff = ff_current();
   if (!ff)
     return hipErrorNotInitialized;
-  const unsigned cols = cfg.columns_count, L = cfg.log_rows_count;
   if ((cols != 3 && cols != 4) || L > 28)
     return hipErrorInvalidValue;
-  hipStream_t s = to_stream(cfg.stream);
-  hipMemPool_t pool = to_pool(cfg.mem_pool);
+  hipStream_t s = ar.stream();
   const unsigned n = cols << L;
   const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
   uint32_t *keys = nullptr, *cells = nullptr, *skeys = nullptr, *scells = nullptr, *flags = nullptr, *excl = nullptr;
   uint32_t *run_of = nullptr, *run_start = nullptr, *run_end = nullptr;
-  BC_TRY(pool_alloc(&keys, n, pool, s));
-  BC_TRY(pool_alloc(&cells, n, pool, s));
-  BC_TRY(pool_alloc(&skeys, n, pool, s));
-  BC_TRY(pool_alloc(&scells, n, pool, s));
-  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(cfg.indexes, keys, cells, cols, L);
+  BC_TRY(ar.get(&keys, n));
+  BC_TRY(ar.get(&cells, n));
+  BC_TRY(ar.get(&skeys, n));
+  BC_TRY(ar.get(&scells, n));
+  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, keys, cells, cols, L);
   BC_LAUNCH_CHECK();
-  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, 0, bits, pool, s));
-  BC_TRY(pool_free(keys, s));
-  BC_TRY(pool_free(cells, s));
-  BC_TRY(pool_alloc(&flags, n, pool, s));
-  BC_TRY(pool_alloc(&excl, n, pool, s));
+  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, 0, bits, ar));
+  BC_TRY(ar.put(keys));
+  BC_TRY(ar.put(cells));
+  BC_TRY(ar.get(&flags, n));
+  BC_TRY(ar.get(&excl, n));
   pn_run_flags_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(skeys, flags, n);
   BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(flags, excl, n, pool, s));
-  BC_TRY(pool_alloc(&run_of, n, pool, s));
-  BC_TRY(pool_alloc(&run_start, n, pool, s));
-  BC_TRY(pool_alloc(&run_end, n, pool, s));
+  BC_TRY(scan_u32_exclusive(flags, excl, n, ar));
+  BC_TRY(ar.get(&run_of, n));
+  BC_TRY(ar.get(&run_start, n));
+  BC_TRY(ar.get(&run_end, n));
   pn_runs_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(flags, excl, run_of, run_start, run_end, n);
   BC_LAUNCH_CHECK();
-  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, static_cast<const fe *>(cfg.scalars), scells, run_of, run_start, run_end,
-                                                                static_cast<fe *>(cfg.target), n, L);
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, scells, run_of, run_start, run_end, target, n, L);
   BC_LAUNCH_CHECK();
   for (uint32_t *p : {skeys, scells, flags, excl, run_of, run_start, run_end})
-    BC_TRY(pool_free(p, s));
+    BC_TRY(ar.put(p));
   return hipSuccess;
 }
 
@@ -114,7 +113,11 @@ extern "C" {
 
 bc_error pn_set_up() { return ff_current() ? bc_success : bc_error_invalid_value; }
 
-bc_error pn_generate_permutation_polynomials(generate_permutation_polynomials_configuration configuration) { return as_bc(pn_generate(configuration)); }
+bc_error pn_generate_permutation_polynomials(generate_permutation_polynomials_configuration configuration) {
+  arena ar = arena::pool_mode(to_pool(configuration.mem_pool), to_stream(configuration.stream));
+  return as_bc(pn_generate_device(configuration.indexes, static_cast<const fe *>(configuration.scalars), static_cast<fe *>(configuration.target),
+                                  configuration.columns_count, configuration.log_rows_count, ar));
+}
 
 bc_error pn_tear_down() { return bc_success; }
 

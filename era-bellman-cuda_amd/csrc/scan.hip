@@ -53,7 +53,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void scan_u32_fixup_kernel(uint32_t *ou
 }
 
 // out[i] = sum_{j<i} in[i]; in and out may alias.
-hipError_t scan_u32_exclusive(const uint32_t *in, uint32_t *out, size_t n, hipMemPool_t pool, hipStream_t stream) {
+hipError_t scan_u32_exclusive(const uint32_t *in, uint32_t *out, size_t n, arena &ar) {
+  hipStream_t stream = ar.stream();
   if (n == 0)
     return hipSuccess;
   const size_t blocks = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -62,13 +63,13 @@ hipError_t scan_u32_exclusive(const uint32_t *in, uint32_t *out, size_t n, hipMe
     return hipGetLastError();
   }
   uint32_t *totals = nullptr;
-  BC_TRY(pool_alloc(&totals, blocks, pool, stream));
+  BC_TRY(ar.get(&totals, blocks));
   scan_u32_block_kernel<<<(unsigned)blocks, SCAN_BLOCK, 0, stream>>>(in, out, totals, n);
   BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(totals, totals, blocks, pool, stream));
+  BC_TRY(scan_u32_exclusive(totals, totals, blocks, ar));
   scan_u32_fixup_kernel<<<(unsigned)blocks, SCAN_BLOCK, 0, stream>>>(out, totals, n);
   BC_LAUNCH_CHECK();
-  return pool_free(totals, stream);
+  return ar.put(totals);
 }
 
 // ---------------- Fr inclusive prefix product ----------------
@@ -119,7 +120,8 @@ __global__ __launch_bounds__(SCAN_BLOCK) void prefix_product_fixup_kernel(fe *ou
 }
 
 // out[i] = prod_{j<=i} in[j]; in and out may alias.
-hipError_t prefix_product(const fe *in, fe *out, size_t n, hipMemPool_t pool, hipStream_t stream) {
+hipError_t prefix_product(const fe *in, fe *out, size_t n, arena &ar) {
+  hipStream_t stream = ar.stream();
   if (n == 0)
     return hipSuccess;
   const size_t blocks = (n + SCAN_TILE - 1) / SCAN_TILE;
@@ -128,13 +130,13 @@ hipError_t prefix_product(const fe *in, fe *out, size_t n, hipMemPool_t pool, hi
     return hipGetLastError();
   }
   fe *totals = nullptr;
-  BC_TRY(pool_alloc(&totals, blocks, pool, stream));
+  BC_TRY(ar.get(&totals, blocks));
   prefix_product_block_kernel<<<(unsigned)blocks, SCAN_BLOCK, 0, stream>>>(in, out, totals, n);
   BC_LAUNCH_CHECK();
-  BC_TRY(prefix_product(totals, totals, blocks, pool, stream));
+  BC_TRY(prefix_product(totals, totals, blocks, ar));
   prefix_product_fixup_kernel<<<(unsigned)blocks, SCAN_BLOCK, 0, stream>>>(out, totals, n);
   BC_LAUNCH_CHECK();
-  return pool_free(totals, stream);
+  return ar.put(totals);
 }
 
 // ---------------- Fr sum ----------------
@@ -161,18 +163,19 @@ __global__ __launch_bounds__(SCAN_BLOCK) void field_sum_kernel(const fe *in, fe 
 }
 
 // *out = sum of in[0..n) (zero for n = 0); in is not modified.
-hipError_t field_sum(const fe *in, fe *out, size_t n, hipMemPool_t pool, hipStream_t stream) {
+hipError_t field_sum(const fe *in, fe *out, size_t n, arena &ar) {
+  hipStream_t stream = ar.stream();
   const size_t blocks = n == 0 ? 1 : (n + SCAN_TILE - 1) / SCAN_TILE;
   if (blocks == 1) {
     field_sum_kernel<<<1, SCAN_BLOCK, 0, stream>>>(in, out, n);
     return hipGetLastError();
   }
   fe *partial = nullptr;
-  BC_TRY(pool_alloc(&partial, blocks, pool, stream));
+  BC_TRY(ar.get(&partial, blocks));
   field_sum_kernel<<<(unsigned)blocks, SCAN_BLOCK, 0, stream>>>(in, partial, n);
   BC_LAUNCH_CHECK();
-  BC_TRY(field_sum(partial, out, blocks, pool, stream));
-  return pool_free(partial, stream);
+  BC_TRY(field_sum(partial, out, blocks, ar));
+  return ar.put(partial);
 }
 
 } // namespace bc

@@ -1,0 +1,352 @@
+// Request loop with continuous batching (extension API of include/bellman-amd.h;
+// no counterpart in the reference).
+//
+// Clients submit MSM / NTT / PN jobs whose buffers already live on the device
+// (bases, twiddles and power tables stay resident between jobs).  A loop
+// thread owns `slots` streams.  Whenever a slot is free it admits the longest
+// FIFO prefix of waiting jobs whose summed footprint fits the HBM budget left
+// by the running batches and launches it as one batch on that slot's stream:
+// MSMs and NTTs as batched launches, PN jobs one after another.  Batches on
+// different slots run concurrently, so jobs that arrive while others are on the
+// GPU start as soon as a slot and memory free up.  Completion is observed by
+// polling each batch's event.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "batch.h"
+#include "common.h"
+#include "pn.h"
+
+namespace bc {
+
+namespace {
+
+uint64_t now_ns() { return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+
+constexpr unsigned SERVER_MAX_PN = 8; // PN jobs per batch
+
+struct request {
+  uint64_t ticket = 0;
+  bc_amd_job_kind kind = BC_AMD_JOB_MSM;
+  msm_job msm{};
+  ntt_job ntt{};
+  bc_amd_pn_job pn{};
+  size_t footprint = 0;
+  bc_amd_job_stats stats{};
+  bool done = false;
+};
+
+struct batch {
+  unsigned slot = 0;
+  std::vector<request *> reqs;
+  size_t bytes = 0;
+};
+
+hipError_t job_footprint(const bc_amd_job &job, request *r) {
+  r->kind = job.kind;
+  switch (job.kind) {
+  case BC_AMD_JOB_MSM:
+    BC_TRY(resolve_msm(job.msm, &r->msm));
+    r->footprint = msm_footprint(r->msm);
+    return hipSuccess;
+  case BC_AMD_JOB_NTT: {
+    const ntt_job n = resolve_ntt(job.ntt);
+    if (n.log_n == 0 || n.log_n > NTT_MAX_LOG || n.log_n + n.log_ext > NTT_MAX_LOG || !n.in || !n.out || (n.log_ext && n.coset >= (1u << n.log_ext)))
+      return hipErrorInvalidValue;
+    r->ntt = n;
+    r->footprint = ntt_footprint(n);
+    return hipSuccess;
+  }
+  case BC_AMD_JOB_PN:
+    if ((job.pn.columns_count != 3 && job.pn.columns_count != 4) || job.pn.log_rows_count > 28 || !job.pn.indexes || !job.pn.scalars || !job.pn.target)
+      return hipErrorInvalidValue;
+    r->pn = job.pn;
+    r->footprint = pn_footprint(job.pn.columns_count, job.pn.log_rows_count);
+    return hipSuccess;
+  }
+  return hipErrorInvalidValue;
+}
+
+class server {
+public:
+  server(int device, size_t budget, unsigned slots) : device_(device), budget_(budget), slots_(slots) {}
+
+  hipError_t start() {
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device_;
+    BC_TRY(hipMemPoolCreate(&pool_, &props));
+    uint64_t threshold = UINT64_MAX; // keep freed memory reserved for the next batches
+    BC_TRY(hipMemPoolSetAttribute(pool_, hipMemPoolAttrReleaseThreshold, &threshold));
+    streams_.resize(slots_, nullptr);
+    events_.resize(slots_, nullptr);
+    busy_.assign(slots_, false);
+    for (unsigned i = 0; i < slots_; i++) {
+      BC_TRY(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
+      BC_TRY(hipEventCreateWithFlags(&events_[i], hipEventDisableTiming));
+    }
+    thread_ = std::thread([this] { loop(); });
+    return hipSuccess;
+  }
+
+  ~server() {
+    if (thread_.joinable()) {
+      {
+        std::lock_guard<std::mutex> lk(m_);
+        stop_ = true;
+      }
+      cv_work_.notify_all();
+      thread_.join();
+    }
+    for (hipStream_t s : streams_)
+      if (s)
+        (void)hipStreamDestroy(s);
+    for (hipEvent_t e : events_)
+      if (e)
+        (void)hipEventDestroy(e);
+    if (pool_)
+      (void)hipMemPoolDestroy(pool_);
+    for (auto &kv : all_)
+      delete kv.second;
+  }
+
+  hipError_t submit(const bc_amd_job &job, uint64_t *ticket) {
+    request *r = new request;
+    const hipError_t e = job_footprint(job, r);
+    if (e != hipSuccess) {
+      delete r;
+      return e;
+    }
+    r->stats.submit_ns = now_ns();
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      if (stop_) {
+        delete r;
+        return hipErrorInvalidValue;
+      }
+      r->ticket = next_ticket_++;
+      all_[r->ticket] = r;
+      pending_.push_back(r);
+      *ticket = r->ticket;
+    }
+    cv_work_.notify_all();
+    return hipSuccess;
+  }
+
+  hipError_t wait(uint64_t ticket, bc_amd_job_stats *stats) {
+    std::unique_lock<std::mutex> lk(m_);
+    auto it = all_.find(ticket);
+    if (it == all_.end())
+      return hipErrorInvalidValue;
+    request *r = it->second;
+    cv_done_.wait(lk, [&] { return r->done; });
+    if (stats)
+      *stats = r->stats;
+    all_.erase(it);
+    delete r;
+    return hipSuccess;
+  }
+
+private:
+  // Longest FIFO prefix that fits the free memory and the per-launch limits.
+  bool take_batch(batch &b) {
+    size_t bytes = 0;
+    unsigned n_msm = 0, n_ntt = 0, n_pn = 0;
+    uint64_t entries = 0;
+    while (!pending_.empty()) {
+      request *r = pending_.front();
+      const bool alone = inflight_.empty() && b.reqs.empty();
+      if (!alone && used_ + bytes + r->footprint > budget_)
+        break;
+      if (r->kind == BC_AMD_JOB_MSM) {
+        const uint64_t e = (uint64_t)((254 - 1) / r->msm.c + 1) << r->msm.log_n;
+        if (n_msm == MSM_MAX_JOBS || (n_msm && entries + e >= (1ull << 31)))
+          break;
+        entries += e;
+        n_msm++;
+      } else if (r->kind == BC_AMD_JOB_NTT) {
+        if (n_ntt == NTT_MAX_JOBS)
+          break;
+        n_ntt++;
+      } else {
+        if (n_pn == SERVER_MAX_PN)
+          break;
+        n_pn++;
+      }
+      pending_.pop_front();
+      bytes += r->footprint;
+      b.reqs.push_back(r);
+    }
+    b.bytes = bytes;
+    return !b.reqs.empty();
+  }
+
+  hipError_t launch(batch &b) {
+    hipStream_t s = streams_[b.slot];
+    std::vector<msm_job> msm;
+    std::vector<ntt_job> ntt;
+    std::vector<const bc_amd_pn_job *> pn;
+    const uint64_t t = now_ns();
+    for (request *r : b.reqs) {
+      r->stats.start_ns = t;
+      r->stats.batch_jobs = (unsigned)b.reqs.size();
+      if (r->kind == BC_AMD_JOB_MSM)
+        msm.push_back(r->msm);
+      else if (r->kind == BC_AMD_JOB_NTT)
+        ntt.push_back(r->ntt);
+      else
+        pn.push_back(&r->pn);
+    }
+    arena ar = arena::pool_mode(pool_, s);
+    BC_TRY(msm_run(msm, ar));
+    BC_TRY(ntt_run(ntt, ar));
+    for (const bc_amd_pn_job *p : pn)
+      BC_TRY(pn_generate_device(p->indexes, static_cast<const fe *>(p->scalars), static_cast<fe *>(p->target), p->columns_count, p->log_rows_count, ar));
+    return hipEventRecord(events_[b.slot], s);
+  }
+
+  void finish(batch &b, hipError_t status) {
+    const uint64_t t = now_ns();
+    for (request *r : b.reqs) {
+      r->stats.done_ns = t;
+      r->stats.status = as_bc(status);
+      r->done = true;
+    }
+    used_ -= b.bytes;
+    busy_[b.slot] = false;
+  }
+
+  void loop() {
+    (void)hipSetDevice(device_);
+    std::unique_lock<std::mutex> lk(m_);
+    for (;;) {
+      bool progress = false;
+      for (size_t i = 0; i < inflight_.size();) {
+        const hipError_t q = hipEventQuery(events_[inflight_[i].slot]);
+        if (q == hipErrorNotReady) {
+          i++;
+          continue;
+        }
+        if (q != hipSuccess)
+          (void)hipStreamSynchronize(streams_[inflight_[i].slot]);
+        finish(inflight_[i], q);
+        inflight_.erase(inflight_.begin() + i);
+        progress = true;
+      }
+      if (progress)
+        cv_done_.notify_all();
+      int slot = -1;
+      for (unsigned i = 0; i < slots_; i++)
+        if (!busy_[i]) {
+          slot = (int)i;
+          break;
+        }
+      if (slot >= 0 && !pending_.empty()) {
+        batch b;
+        b.slot = (unsigned)slot;
+        if (take_batch(b)) {
+          used_ += b.bytes;
+          busy_[b.slot] = true;
+          lk.unlock();
+          hipError_t e = launch(b);
+          if (e != hipSuccess)
+            (void)hipStreamSynchronize(streams_[b.slot]); // work already enqueued must end before its memory is reused
+          lk.lock();
+          if (e != hipSuccess) {
+            finish(b, e);
+            cv_done_.notify_all();
+          } else {
+            inflight_.push_back(std::move(b));
+          }
+          continue;
+        }
+      }
+      if (stop_ && pending_.empty() && inflight_.empty())
+        break;
+      if (inflight_.empty())
+        cv_work_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+      else
+        cv_work_.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+
+  int device_;
+  size_t budget_;
+  unsigned slots_;
+  size_t used_ = 0;
+  hipMemPool_t pool_ = nullptr;
+  std::vector<hipStream_t> streams_;
+  std::vector<hipEvent_t> events_;
+  std::vector<bool> busy_;
+  std::mutex m_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<request *> pending_;
+  std::unordered_map<uint64_t, request *> all_;
+  std::vector<batch> inflight_;
+  uint64_t next_ticket_ = 1;
+  bool stop_ = false;
+  std::thread thread_;
+};
+
+} // namespace
+
+} // namespace bc
+
+using namespace bc;
+
+extern "C" {
+
+bc_error bc_amd_job_footprint(const bc_amd_job *job, size_t *bytes) {
+  if (!job || !bytes)
+    return bc_error_invalid_value;
+  request r;
+  BC_TRY_BC(job_footprint(*job, &r));
+  *bytes = r.footprint;
+  return bc_success;
+}
+
+bc_error bc_amd_server_create(bc_amd_server *server_out, size_t hbm_budget_bytes, unsigned max_in_flight) {
+  if (!server_out || max_in_flight == 0 || max_in_flight > 16 || hbm_budget_bytes == 0)
+    return bc_error_invalid_value;
+  int device = 0;
+  BC_TRY_BC(hipGetDevice(&device));
+  server *s = new server(device, hbm_budget_bytes, max_in_flight);
+  const hipError_t e = s->start();
+  if (e != hipSuccess) {
+    delete s;
+    server_out->handle = nullptr;
+    return as_bc(e);
+  }
+  server_out->handle = s;
+  return bc_success;
+}
+
+bc_error bc_amd_server_submit(bc_amd_server srv, const bc_amd_job *job, unsigned long long *ticket) {
+  if (!srv.handle || !job || !ticket)
+    return bc_error_invalid_value;
+  uint64_t t = 0;
+  BC_TRY_BC(static_cast<server *>(srv.handle)->submit(*job, &t));
+  *ticket = t;
+  return bc_success;
+}
+
+bc_error bc_amd_server_wait(bc_amd_server srv, unsigned long long ticket, bc_amd_job_stats *stats) {
+  if (!srv.handle)
+    return bc_error_invalid_value;
+  return as_bc(static_cast<server *>(srv.handle)->wait(ticket, stats));
+}
+
+bc_error bc_amd_server_destroy(bc_amd_server srv) {
+  if (!srv.handle)
+    return bc_error_invalid_value;
+  delete static_cast<server *>(srv.handle);
+  return bc_success;
+}
+
+} // extern "C"

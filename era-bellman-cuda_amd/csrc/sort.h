@@ -7,10 +7,9 @@
 
 namespace bc {
 
-hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned begin_bit, unsigned end_bit, hipMemPool_t pool,
-                           hipStream_t stream);
+hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned begin_bit, unsigned end_bit, arena &ar);
 
 hipError_t radix_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, size_t n, unsigned begin_bit,
-                            unsigned end_bit, hipMemPool_t pool, hipStream_t stream);
+                            unsigned end_bit, arena &ar);
 
 } // namespace bc

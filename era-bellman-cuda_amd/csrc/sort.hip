@@ -90,7 +90,8 @@ __global__ __launch_bounds__(RS_BLOCK) void rs_scatter_kernel(const uint32_t *ke
 }
 
 hipError_t radix_sort_impl(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *vals_in, uint32_t *vals_out, size_t n, unsigned begin_bit,
-                           unsigned end_bit, hipMemPool_t pool, hipStream_t stream) {
+                           unsigned end_bit, arena &ar) {
+  hipStream_t stream = ar.stream();
   if (n == 0)
     return hipSuccess;
   const bool with_values = vals_in != nullptr;
@@ -103,10 +104,10 @@ hipError_t radix_sort_impl(const uint32_t *keys_in, uint32_t *keys_out, const ui
   const unsigned passes = (end_bit - begin_bit + 7) / 8;
   const unsigned tiles = (unsigned)((n + RS_TILE - 1) / RS_TILE);
   uint32_t *table = nullptr, *tmp_k = nullptr, *tmp_v = nullptr;
-  BC_TRY(pool_alloc(&table, (size_t)RS_DIGITS * tiles, pool, stream));
-  BC_TRY(pool_alloc(&tmp_k, n, pool, stream));
+  BC_TRY(ar.get(&table, (size_t)RS_DIGITS * tiles));
+  BC_TRY(ar.get(&tmp_k, n));
   if (with_values)
-    BC_TRY(pool_alloc(&tmp_v, n, pool, stream));
+    BC_TRY(ar.get(&tmp_v, n));
   // ping-pong so that the last pass lands in *_out
   const uint32_t *src_k = keys_in, *src_v = vals_in;
   for (unsigned p = 0; p < passes; p++) {
@@ -119,7 +120,7 @@ hipError_t radix_sort_impl(const uint32_t *keys_in, uint32_t *keys_out, const ui
     (void)last;
     rs_histogram_kernel<<<tiles, RS_BLOCK, 0, stream>>>(src_k, n, shift, bits, table, tiles);
     BC_LAUNCH_CHECK();
-    BC_TRY(scan_u32_exclusive(table, table, (size_t)RS_DIGITS * tiles, pool, stream));
+    BC_TRY(scan_u32_exclusive(table, table, (size_t)RS_DIGITS * tiles, ar));
     if (with_values)
       rs_scatter_kernel<true><<<tiles, RS_BLOCK, 0, stream>>>(src_k, dst_k, src_v, dst_v, n, shift, bits, table, tiles);
     else
@@ -128,23 +129,22 @@ hipError_t radix_sort_impl(const uint32_t *keys_in, uint32_t *keys_out, const ui
     src_k = dst_k;
     src_v = dst_v;
   }
-  BC_TRY(pool_free(table, stream));
-  BC_TRY(pool_free(tmp_k, stream));
+  BC_TRY(ar.put(table));
+  BC_TRY(ar.put(tmp_k));
   if (with_values)
-    BC_TRY(pool_free(tmp_v, stream));
+    BC_TRY(ar.put(tmp_v));
   return hipSuccess;
 }
 
 } // namespace
 
-hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned begin_bit, unsigned end_bit, hipMemPool_t pool,
-                           hipStream_t stream) {
-  return radix_sort_impl(keys_in, keys_out, nullptr, nullptr, n, begin_bit, end_bit, pool, stream);
+hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned begin_bit, unsigned end_bit, arena &ar) {
+  return radix_sort_impl(keys_in, keys_out, nullptr, nullptr, n, begin_bit, end_bit, ar);
 }
 
 hipError_t radix_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, size_t n, unsigned begin_bit,
-                            unsigned end_bit, hipMemPool_t pool, hipStream_t stream) {
-  return radix_sort_impl(keys_in, keys_out, values_in, values_out, n, begin_bit, end_bit, pool, stream);
+                            unsigned end_bit, arena &ar) {
+  return radix_sort_impl(keys_in, keys_out, values_in, values_out, n, begin_bit, end_bit, ar);
 }
 
 } // namespace bc
