@@ -131,6 +131,69 @@ class ntt_configuration(ctypes.Structure):
     ]
 
 
+# ---------------------------------------------------------------- extensions (include/bellman-amd.h)
+class bc_amd_msm_job(ctypes.Structure):
+    _fields_ = [
+        ("bases", ctypes.c_void_p),
+        ("scalars", ctypes.c_void_p),
+        ("results", ctypes.c_void_p),
+        ("log_scalars_count", ctypes.c_uint),
+        ("window_bits", ctypes.c_uint),
+        ("piece", ctypes.c_uint),
+    ]
+
+
+class bc_amd_ntt_job(ctypes.Structure):
+    _fields_ = [
+        ("inputs", ctypes.c_void_p),
+        ("outputs", ctypes.c_void_p),
+        ("log_values_count", ctypes.c_uint),
+        ("bit_reversed_inputs", ctypes.c_bool),
+        ("inverse", ctypes.c_bool),
+        ("log_extension_degree", ctypes.c_uint),
+        ("coset_index", ctypes.c_uint),
+    ]
+
+
+class bc_amd_pn_job(ctypes.Structure):
+    _fields_ = [
+        ("indexes", ctypes.c_void_p),
+        ("scalars", ctypes.c_void_p),
+        ("target", ctypes.c_void_p),
+        ("columns_count", ctypes.c_uint),
+        ("log_rows_count", ctypes.c_uint),
+    ]
+
+
+BC_AMD_JOB_MSM, BC_AMD_JOB_NTT, BC_AMD_JOB_PN = 0, 1, 2
+
+
+class bc_amd_job(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("msm", bc_amd_msm_job), ("ntt", bc_amd_ntt_job), ("pn", bc_amd_pn_job)]
+
+
+class bc_amd_job_stats(ctypes.Structure):
+    _fields_ = [
+        ("submit_ns", ctypes.c_ulonglong),
+        ("start_ns", ctypes.c_ulonglong),
+        ("done_ns", ctypes.c_ulonglong),
+        ("batch_jobs", ctypes.c_uint),
+        ("status", ctypes.c_int),
+    ]
+
+
+class bc_amd_msm_tuning(ctypes.Structure):
+    _fields_ = [("window_bits", ctypes.c_uint), ("piece", ctypes.c_uint)]
+
+
+class bc_amd_plan(ctypes.Structure):
+    _fields_ = [("handle", ctypes.c_void_p)]
+
+
+class bc_amd_server(ctypes.Structure):
+    _fields_ = [("handle", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -173,7 +236,12 @@ HEADER_SYMBOLS = [
 ]
 
 # Extension entry points (include/bellman-amd.h) with no counterpart in the reference.
-EXTENSION_SYMBOLS = ["bc_amd_generator_multiples"]
+EXTENSION_SYMBOLS = [
+    "bc_amd_generator_multiples", "bc_amd_msm_batch_async", "bc_amd_ntt_batch_async", "bc_amd_msm_get_tuning", "bc_amd_msm_set_tuning",
+    "bc_amd_ntt_get_split", "bc_amd_ntt_set_split", "bc_amd_msm_autotune", "bc_amd_ntt_autotune", "bc_amd_tuning_save",
+    "bc_amd_tuning_load", "bc_amd_plan_create", "bc_amd_plan_workspace_bytes", "bc_amd_plan_launch", "bc_amd_plan_destroy",
+    "bc_amd_job_footprint", "bc_amd_server_create", "bc_amd_server_submit", "bc_amd_server_wait", "bc_amd_server_destroy",
+]
 
 
 # ---------------------------------------------------------------- helpers
@@ -223,3 +291,41 @@ class Device:
 
 def ptr(a):
     return ctypes.c_void_p(a) if isinstance(a, int) else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def msm_job(bases, scalars, results, log_n, window_bits=0, piece=0):
+    return bc_amd_msm_job(bases, scalars, results, log_n, window_bits, piece)
+
+
+def ntt_job(inputs, outputs, log_n, bitrev=False, inverse=False, log_ext=0, coset=0):
+    return bc_amd_ntt_job(inputs, outputs, log_n, bitrev, inverse, log_ext, coset)
+
+
+def array_of(ctype, items):
+    arr = (ctype * max(len(items), 1))()
+    for i, it in enumerate(items):
+        arr[i] = it
+    return arr
+
+
+class Server:
+    """The continuous-batching request loop (bc_amd_server_*)."""
+
+    def __init__(self, hbm_budget_bytes, max_in_flight=2):
+        self.handle = bc_amd_server()
+        call("bc_amd_server_create", ctypes.byref(self.handle), ctypes.c_size_t(int(hbm_budget_bytes)), ctypes.c_uint(max_in_flight))
+
+    def submit(self, job):
+        t = ctypes.c_ulonglong()
+        call("bc_amd_server_submit", self.handle, ctypes.byref(job), ctypes.byref(t))
+        return t.value
+
+    def wait(self, ticket):
+        st = bc_amd_job_stats()
+        call("bc_amd_server_wait", self.handle, ctypes.c_ulonglong(ticket), ctypes.byref(st))
+        return st
+
+    def close(self):
+        if self.handle.handle:
+            call("bc_amd_server_destroy", self.handle)
+            self.handle.handle = None

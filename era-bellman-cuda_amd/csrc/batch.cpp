@@ -29,7 +29,7 @@ struct tuning_table {
   unsigned ntt_first[TUNE_MAX_LOG + 1];
   tuning_table() {
     for (unsigned l = 0; l <= TUNE_MAX_LOG; l++) {
-      msm[l] = {msm_window_bits(l), MSM_DEFAULT_PIECE};
+      msm[l] = {std::min(msm_window_bits(l), 22u), MSM_DEFAULT_PIECE};
       ntt_first[l] = 0;
     }
   }

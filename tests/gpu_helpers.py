@@ -85,3 +85,47 @@ def ff_binary(dev, name, a, x, y, a_on_device=False):
         if p is not None:
             dev.free(p)
     return out
+
+
+def pn_cycles(cols, log_rows, seed):
+    """Variable indexes forming random cycles of length 1..64 over the cells."""
+    import random
+
+    cells = cols << log_rows
+    rng = random.Random(seed)
+    perm = list(range(cells))
+    rng.shuffle(perm)
+    idx = np.zeros(cells, dtype=np.uint32)
+    pos = var = 0
+    while pos < cells:
+        L = rng.randint(1, 64)
+        idx[perm[pos:pos + L]] = var
+        pos += L
+        var += 1
+    return idx
+
+
+def pn(dev, idx, ks, cols, log_rows):
+    """pn_generate_permutation_polynomials on device copies; returns the target."""
+    di, dk = dev.upload(idx), dev.upload(ks)
+    dt = dev.malloc((cols << log_rows) * 32)
+    cfg = B.generate_permutation_polynomials_configuration(dev.pool, dev.stream, di, dk, dt, cols, log_rows)
+    B.call("pn_generate_permutation_polynomials", cfg)
+    dev.sync()
+    out = dev.download(dt, (cols << log_rows, 8))
+    for p in (di, dk, dt):
+        dev.free(p)
+    return out
+
+
+def msm_batch(dev, jobs):
+    """bc_amd_msm_batch_async over a list of bc_amd_msm_job; waits for the stream."""
+    arr = B.array_of(B.bc_amd_msm_job, jobs)
+    B.call("bc_amd_msm_batch_async", arr, ctypes.c_uint(len(jobs)), dev.pool, dev.stream)
+    dev.sync()
+
+
+def ntt_batch(dev, jobs):
+    arr = B.array_of(B.bc_amd_ntt_job, jobs)
+    B.call("bc_amd_ntt_batch_async", arr, ctypes.c_uint(len(jobs)), dev.pool, dev.stream)
+    dev.sync()

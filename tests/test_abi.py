@@ -8,11 +8,18 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "bellman-cuda.h")
+EXT_HEADER = os.path.join(ROOT, "include", "bellman-amd.h")
 
 
-def header_functions():
-    text = open(HEADER).read()
+def header_functions(path=HEADER):
+    text = open(path).read()
     return re.findall(r"^bc_error\s+(\w+)\s*\(", text, flags=re.M)
+
+
+def test_extension_header_matches_binding():
+    import bellman_amd as B
+
+    assert sorted(header_functions(EXT_HEADER)) == sorted(B.EXTENSION_SYMBOLS)
 
 
 def test_header_declares_the_reference_surface():
@@ -65,3 +72,28 @@ int main(void) {
                                         B.ff_poly_evaluate_configuration, B.ff_sort_u32_configuration,
                                         B.generate_permutation_polynomials_configuration, B.msm_configuration, B.ntt_configuration)]
     assert sizes == mine
+
+
+def test_extension_struct_layouts_match_c_abi():
+    import ctypes
+    import tempfile
+
+    import bellman_amd as B
+
+    names = ["bc_amd_msm_job", "bc_amd_ntt_job", "bc_amd_pn_job", "bc_amd_job", "bc_amd_job_stats", "bc_amd_msm_tuning", "bc_amd_plan",
+             "bc_amd_server"]
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"bellman-amd.h\"\nint main(void) {\n"
+    for n in names:
+        src += '  printf("%%zu ", sizeof(%s));\n' % n
+    src += '  printf("%zu %zu %zu ", offsetof(bc_amd_job, msm), offsetof(bc_amd_job, ntt), offsetof(bc_amd_job, pn));\n'
+    src += '  printf("%zu %zu\\n", offsetof(bc_amd_ntt_job, log_extension_degree), offsetof(bc_amd_job_stats, status));\n  return 0;\n}\n'
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "s.c")
+        open(c, "w").write(src)
+        exe = os.path.join(d, "s")
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        got = [int(v) for v in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    mine = [ctypes.sizeof(getattr(B, n)) for n in names]
+    mine += [B.bc_amd_job.msm.offset, B.bc_amd_job.ntt.offset, B.bc_amd_job.pn.offset, B.bc_amd_ntt_job.log_extension_degree.offset,
+             B.bc_amd_job_stats.status.offset]
+    assert got == mine
