@@ -1,22 +1,31 @@
 """Benchmark of the prover hot path on one MI355X.
 
-A step is one BN254 G1 MSM of 2^24 points plus one Fr NTT of 2^24 elements
-(forward, natural -> bit-reversed), both called through the bellman-cuda C ABI
-with inputs resident in HBM.  Inputs are synthetic and seeded: bases
-[s_i]G for uniform s_i (generated on the device by a fixed-base comb), MSM
-scalars 70% uniform / 20% < 2^64 / 10% in {0,1}, uniform NTT inputs.
-Prints one JSON line.  The CPU baseline is the C oracle timed on a bounded
-sample on the host.
+Headline step: one BN254 G1 MSM of 2^24 points plus one Fr NTT of 2^24
+elements (forward, natural -> bit-reversed), both through the bellman-cuda C
+ABI with inputs resident in HBM.  Beside it:
+  batched: 64 MSMs (2^12..2^20, shared 2^20 bases) + 32 NTTs (2^10..2^22) as one
+           captured plan, against the same 96 jobs issued one at a time through
+           the plain C ABI with a stream synchronisation after each;
+  served:  a mixed stream of MSM (2^14..2^22), NTT (2^12..2^22) and PN
+           (4 x 2^20) jobs through the continuous-batching server: capacity,
+           p50/p99 latency under Poisson arrivals, and the one-at-a-time FIFO
+           under the same arrivals.
+Every batched and served result is checked against its plain-call result (NTT
+and PN bytes, MSM folded point) and every MSM fold against the closed form
+[sum k_i s_i]G, on the device.  Inputs are synthetic and seeded (splitmix64).
+Prints one JSON line; progress goes to stderr.
 """
 import argparse
 import ctypes
+import hashlib
 import json
 import os
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
-for p in (ROOT, os.path.join(ROOT, "era-bellman-cuda_amd"), os.path.join(ROOT, "tests")):
+for p in (ROOT, os.path.join(ROOT, "era-bellman-cuda_amd")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
@@ -25,20 +34,87 @@ import numpy as np  # noqa: E402
 import bellman_amd as B  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# v_mad_u64_u32 lane-operations per second measured on MI355X by a register-only
+# microbenchmark (256 CUs); the issue-rate ceiling of the MSM's 32x32->64 products.
+VALU_MAD_PEAK_T = 31.0
+R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
+MSM_METRIC_LOG = 24
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-class Timer:
-    def __init__(self, dev):
-        self.dev = dev
+# ---------------------------------------------------------------- synthetic inputs
+M64 = (1 << 64) - 1
+R_LIMBS = [(R >> (64 * i)) & M64 for i in range(4)]
+
+
+def splitmix(seed, idx):
+    """splitmix64 output number idx of the stream `seed` (vectorised over idx)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (idx.astype(np.uint64) + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_fr(seed, n):
+    """n uniform canonical Fr values as (n, 8) u32 limbs: 254-bit draws, rejected if >= r."""
+    out = np.zeros((n, 4), dtype=np.uint64)
+    todo = np.arange(n)
+    rnd = 0
+    while todo.size:
+        base = (np.uint64(rnd) << np.uint64(40)) + todo.astype(np.uint64) * np.uint64(4)
+        cand = np.stack([splitmix(seed, base + np.uint64(j)) for j in range(4)], axis=1)
+        cand[:, 3] &= np.uint64((1 << 62) - 1)
+        lt = np.zeros(todo.size, dtype=bool)
+        eq = np.ones(todo.size, dtype=bool)
+        for j in (3, 2, 1, 0):
+            lt |= eq & (cand[:, j] < np.uint64(R_LIMBS[j]))
+            eq &= cand[:, j] == np.uint64(R_LIMBS[j])
+        out[todo[lt]] = cand[lt]
+        todo = todo[~lt]
+        rnd += 1
+    return out.view(np.uint32).reshape(n, 8)
+
+
+def msm_scalars_canonical(seed, n):
+    """70% uniform in [0, r), 20% below 2^64, 10% in {0, 1} (canonical limbs)."""
+    idx = np.arange(n)
+    u = splitmix(seed ^ 0x5A5A5A5A, idx) % np.uint64(100)
+    small = splitmix(seed ^ 0xA5A5A5A5, idx)
+    out = uniform_fr(seed, n)
+    out64 = out.view(np.uint64).reshape(n, 4)
+    mid = (u >= 70) & (u < 90)
+    out64[mid] = 0
+    out64[mid, 0] = small[mid]
+    tail = u >= 90
+    out64[tail] = 0
+    out64[tail, 0] = small[tail] & np.uint64(1)
+    return out
+
+
+class Lab:
+    """Device, stream, timer and the device-side helpers the checks use."""
+
+    def __init__(self):
+        B.call("ff_set_up", ctypes.c_uint(25), ctypes.c_uint(14))
+        B.call("ntt_set_up")
+        B.call("pn_set_up")
+        B.call("msm_set_up")
+        self.dev = B.Device(0)
         self.start, self.stop = B.bc_event(), B.bc_event()
         B.call("bc_event_create", ctypes.byref(self.start), ctypes.c_bool(False), ctypes.c_bool(False))
         B.call("bc_event_create", ctypes.byref(self.stop), ctypes.c_bool(False), ctypes.c_bool(False))
+        r2 = (1 << 512) % R
+        self.r2 = np.array([(r2 >> (32 * i)) & 0xFFFFFFFF for i in range(8)], dtype=np.uint32)
+        self.raw_one = np.array([1, 0, 0, 0, 0, 0, 0, 0], dtype=np.uint32)
+        mont_one = (1 << 256) % R
+        self.mont_one = np.array([(mont_one >> (32 * i)) & 0xFFFFFFFF for i in range(8)], dtype=np.uint32)
 
-    def time(self, fn, reps):
+    def time_ms(self, fn, reps=1):
+        """Events on the library's stream around `reps` calls of fn; ms per call."""
         B.call("bc_event_record", self.start, self.dev.stream)
         for _ in range(reps):
             fn()
@@ -48,78 +124,441 @@ class Timer:
         B.call("bc_event_elapsed_time", ctypes.byref(ms), self.start, self.stop)
         return ms.value / reps
 
+    def to_mont(self, d_canon, d_out, n):
+        """Montgomery form on the device: mont_mul(R^2, v) = v R."""
+        B.call("ff_ax", B.ptr(self.r2), ctypes.c_void_p(d_canon), ctypes.c_void_p(d_out), ctypes.c_uint(n), self.dev.stream)
 
-def msm_call(dev, d_bases, d_scalars, d_results, L):
+    def upload_mont(self, canon):
+        d = self.dev.upload(canon)
+        self.to_mont(d, d, canon.shape[0])
+        return d
+
+    def bases(self, seed, n):
+        """[s_i]G for uniform s_i (device), plus the device copy of s_i (canonical)."""
+        s = uniform_fr(seed, n)
+        d_s = self.dev.upload(s)
+        d_b = self.dev.malloc(n * 64)
+        B.call("bc_amd_generator_multiples", ctypes.c_void_p(d_s), ctypes.c_void_p(d_b), ctypes.c_uint(n), self.dev.stream)
+        return d_b, d_s
+
+    def expected_points(self, d_s, k_ptrs, ns):
+        """[sum_i k_i s_i]G for each job (s canonical prefix, k Montgomery), as affine points."""
+        dev = self.dev
+        m = len(k_ptrs)
+        d_t = dev.malloc(32 * m)
+        nmax = max(ns)
+        d_tmp = dev.malloc(32 * nmax)
+        for j, (dk, n) in enumerate(zip(k_ptrs, ns)):
+            self.to_mont(d_s, d_tmp, n)
+            B.call("ff_x_mul_y", ctypes.c_void_p(d_tmp), ctypes.c_void_p(dk), ctypes.c_void_p(d_tmp), ctypes.c_uint(n), dev.stream)
+            cfg = B.ff_poly_evaluate_configuration(dev.pool, dev.stream, d_tmp, self.mont_one.ctypes.data, d_t + 32 * j, n)
+            B.call("ff_poly_evaluate", cfg)
+        # canonical t = mont_mul(1, t R)
+        B.call("ff_ax", B.ptr(self.raw_one), ctypes.c_void_p(d_t), ctypes.c_void_p(d_t), ctypes.c_uint(m), dev.stream)
+        d_p = dev.malloc(64 * m)
+        B.call("bc_amd_generator_multiples", ctypes.c_void_p(d_t), ctypes.c_void_p(d_p), ctypes.c_uint(m), dev.stream)
+        dev.sync()
+        out = dev.download(d_p, (m, 16))
+        for p in (d_t, d_tmp, d_p):
+            dev.free(p)
+        return out
+
+    def folds(self, res_ptrs):
+        """On-device fold of per-bit results to affine points."""
+        dev = self.dev
+        m = len(res_ptrs)
+        arr = (ctypes.c_void_p * m)(*res_ptrs)
+        d_p = dev.malloc(64 * m)
+        B.call("bc_amd_msm_fold", arr, ctypes.c_void_p(d_p), ctypes.c_uint(m), dev.stream)
+        dev.sync()
+        out = dev.download(d_p, (m, 16))
+        dev.free(d_p)
+        return out
+
+
+def msm_call(lab, d_bases, d_scalars, d_results, L):
     cfg = B.msm_configuration()
-    cfg.mem_pool, cfg.stream = dev.pool, dev.stream
+    cfg.mem_pool, cfg.stream = lab.dev.pool, lab.dev.stream
     cfg.bases, cfg.scalars, cfg.results, cfg.log_scalars_count = d_bases, d_scalars, d_results, L
     B.call("msm_execute_async", cfg)
 
 
-def ntt_call(dev, d_in, d_out, L, bitrev=False, inverse=False, log_ext=0, coset=0):
+def ntt_call(lab, d_in, d_out, L, bitrev=False, inverse=False, log_ext=0, coset=0):
     cfg = B.ntt_configuration()
-    cfg.mem_pool, cfg.stream = dev.pool, dev.stream
+    cfg.mem_pool, cfg.stream = lab.dev.pool, lab.dev.stream
     cfg.inputs, cfg.outputs, cfg.log_values_count = d_in, d_out, L
     cfg.bit_reversed_inputs, cfg.inverse = bitrev, inverse
     cfg.log_extension_degree, cfg.coset_index = log_ext, coset
     B.call("ntt_execute_async", cfg)
 
 
-def msm_checked(dev, O, L, steps, warmup, timer):
-    n = 1 << L
+def pn_call(lab, d_idx, d_ks, d_out, cols, log_rows):
+    cfg = B.generate_permutation_polynomials_configuration(lab.dev.pool, lab.dev.stream, d_idx, d_ks, d_out, cols, log_rows)
+    B.call("pn_generate_permutation_polynomials", cfg)
+
+
+def pn_indexes(seed, cols, log_rows):
+    """Variable indexes forming random cycles of length 1..64 over the cells."""
+    cells = cols << log_rows
+    rng = np.random.RandomState(seed & 0xFFFFFFFF)
+    perm = rng.permutation(cells)
+    lengths = rng.randint(1, 65, size=cells // 16 + 64)
+    ends = np.cumsum(lengths)
+    var_of_pos = np.searchsorted(ends, np.arange(cells), side="right")
+    idx = np.zeros(cells, dtype=np.uint32)
+    idx[perm] = var_of_pos.astype(np.uint32)
+    return idx
+
+
+def pct(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+# ---------------------------------------------------------------- autotuning
+def autotune(lab, log_lo=12, log_hi=24):
+    dev = lab.dev
     t0 = time.time()
-    s = O.random_fr(0xBC000002A, n, montgomery=False)
-    k = O.msm_scalars(0xBC000002B, n)
-    log("msm 2^%d: inputs generated on host in %.1fs" % (L, time.time() - t0))
-    d_s = dev.upload(s)
-    d_k = dev.upload(k)
-    d_bases = dev.malloc(n * 64)
-    B.call("bc_amd_generator_multiples", ctypes.c_void_p(d_s), ctypes.c_void_p(d_bases), ctypes.c_uint(n), dev.stream)
-    d_res = dev.malloc(254 * 96)
-    dev.sync()
-    for _ in range(warmup):
-        msm_call(dev, d_bases, d_k, d_res, L)
-    dev.sync()
-    ms = timer.time(lambda: msm_call(dev, d_bases, d_k, d_res, L), steps)
-    res = dev.download(d_res, (254, 24))
-    # size-independent check: fold == [sum k_i s_i] G, the sum taken on the device with ff ops
-    d_sm = dev.malloc(n * 32)
-    r2 = O.to_limbs([(1 << 512) % O.R])
-    B.call("ff_ax", B.ptr(r2), ctypes.c_void_p(d_s), ctypes.c_void_p(d_sm), ctypes.c_uint(n), dev.stream)  # s -> Montgomery
-    B.call("ff_x_mul_y", ctypes.c_void_p(d_sm), ctypes.c_void_p(d_k), ctypes.c_void_p(d_sm), ctypes.c_uint(n), dev.stream)
-    one = O.to_mont([1])
-    tsum = np.zeros((1, 8), dtype=np.uint32)
-    B.call("ff_poly_evaluate", B.ff_poly_evaluate_configuration(dev.pool, dev.stream, d_sm, one.ctypes.data, tsum.ctypes.data, n))
-    dev.sync()
-    t = O.from_mont(tsum)[0]
-    ok = bool(np.array_equal(O.to_affine(O.msm_fold(res))[0], O.mul_generator(t)))
-    for p in (d_s, d_k, d_bases, d_res, d_sm):
+    n = 1 << log_hi
+    d_b, d_s = lab.bases(0xBC000010, n)
+    d_k = lab.upload_mont(msm_scalars_canonical(0xBC000011, n))
+    B.call("bc_amd_msm_autotune", ctypes.c_void_p(d_b), ctypes.c_void_p(d_k), ctypes.c_uint(log_lo), ctypes.c_uint(log_hi), dev.stream)
+    scratch = dev.malloc(32 << 24)
+    B.call("bc_amd_ntt_autotune", ctypes.c_void_p(scratch), ctypes.c_uint(13), ctypes.c_uint(24), dev.stream)
+    for p in (d_b, d_s, d_k, scratch):
         dev.free(p)
-    return ms, ok
+    table = {}
+    for L in range(log_lo, log_hi + 1):
+        t = B.bc_amd_msm_tuning()
+        B.call("bc_amd_msm_get_tuning", ctypes.c_uint(L), ctypes.byref(t))
+        table["msm_%d" % L] = [t.window_bits, t.piece]
+    for L in range(13, 25):
+        f = ctypes.c_uint()
+        B.call("bc_amd_ntt_get_split", ctypes.c_uint(L), ctypes.byref(f))
+        table["ntt_%d" % L] = f.value
+    log("autotune: %.1fs %s" % (time.time() - t0, table))
+    return table
 
 
-def ntt_checked(dev, O, L, steps, warmup, timer):
+# ---------------------------------------------------------------- headline: 2^24 MSM + NTT
+def headline_ntt(lab, L, steps, warmup):
+    dev = lab.dev
     n = 1 << L
-    x = O.random_fr(0xBC000003, n)
-    d_x = dev.upload(x)
+    x = uniform_fr(0xBC000003, n)
+    d_x = lab.upload_mont(x)
     d_y = dev.malloc(n * 32)
     for _ in range(warmup):
-        ntt_call(dev, d_x, d_y, L)
+        ntt_call(lab, d_x, d_y, L)
     dev.sync()
-    ms = timer.time(lambda: ntt_call(dev, d_x, d_y, L), steps)
-    lde_ms = timer.time(lambda: [ntt_call(dev, d_x, d_y, L, log_ext=2, coset=c) for c in range(4)], max(1, steps // 2))
-    # round trip: inverse bit-reversed -> natural brings the input back
-    ntt_call(dev, d_x, d_y, L)
-    ntt_call(dev, d_y, d_y, L, bitrev=True, inverse=True)
-    dev.sync()
-    ok = bool(np.array_equal(dev.download(d_y, (n, 8)), x))
+    ms = lab.time_ms(lambda: ntt_call(lab, d_x, d_y, L), steps)
+    inv_ms = lab.time_ms(lambda: ntt_call(lab, d_y, d_y, L, bitrev=True, inverse=True), 1)
+    # forward then inverse must give the input back (the inverse above ran in place on d_y)
+    ok = bool(np.array_equal(dev.download(d_y, (n, 8)), dev.download(d_x, (n, 8))))
+    lde_ms = lab.time_ms(lambda: [ntt_call(lab, d_x, d_y, L, log_ext=2, coset=c) for c in range(4)], max(1, steps // 2))
     dev.free(d_x)
     dev.free(d_y)
-    return ms, lde_ms, ok
+    return ms, inv_ms, lde_ms, ok
 
 
-def cpu_baseline(O, L_ntt_sample=20, L_msm_sample=16):
-    """The C oracle on the host: NTT 2^20 on one thread, MSM 2^16 (OpenMP over windows)."""
+def headline_msm(lab, L, steps, warmup):
+    dev = lab.dev
+    n = 1 << L
+    t0 = time.time()
+    d_b, d_s = lab.bases(0xBC00002A, n)
+    d_k = lab.upload_mont(msm_scalars_canonical(0xBC00002B, n))
+    log("msm 2^%d: inputs ready in %.1fs" % (L, time.time() - t0))
+    d_r = dev.malloc(254 * 96)
+    for _ in range(warmup):
+        msm_call(lab, d_b, d_k, d_r, L)
+    dev.sync()
+    ms = lab.time_ms(lambda: msm_call(lab, d_b, d_k, d_r, L), steps)
+    got = lab.folds([d_r])[0]
+    exp = lab.expected_points(d_s, [d_k], [n])[0]
+    ok = bool(np.array_equal(got, exp))
+    # the same MSM with the autotuned window width through the batch entry point
+    job = B.msm_job(d_b, d_k, d_r, L)
+    arr = B.array_of(B.bc_amd_msm_job, [job])
+    tuned_ms = lab.time_ms(lambda: B.call("bc_amd_msm_batch_async", arr, ctypes.c_uint(1), dev.pool, dev.stream), steps)
+    tuned_ok = bool(np.array_equal(lab.folds([d_r])[0], exp))
+    for p in (d_b, d_s, d_k, d_r):
+        dev.free(p)
+    return ms, ok, tuned_ms, tuned_ok
+
+
+# ---------------------------------------------------------------- batched (64 MSM + 32 NTT)
+def batched(lab, reps):
+    dev = lab.dev
+    rng = np.random.RandomState(0xBC000004 & 0xFFFFFFFF)
+    msm_logs = [int(v) for v in rng.randint(12, 21, 64)]
+    ntt_logs = [int(v) for v in rng.randint(10, 23, 32)]
+    t0 = time.time()
+    d_b, d_s = lab.bases(0xBC000002, 1 << 20)
+    d_k, r_plain, r_batch = [], [], []
+    for j, L in enumerate(msm_logs):
+        d_k.append(lab.upload_mont(msm_scalars_canonical(0xBC000040 + j, 1 << L)))
+        r_plain.append(dev.malloc(254 * 96))
+        r_batch.append(dev.malloc(254 * 96))
+    x_pool = lab.upload_mont(uniform_fr(0xBC000004, 1 << 23))
+    x_in, y_plain, y_batch = [], [], []
+    for j, L in enumerate(ntt_logs):
+        off = (j * 1000003) % ((1 << 23) - (1 << L) + 1)
+        x_in.append(x_pool + 32 * off)
+        y_plain.append(dev.malloc(32 << L))
+        y_batch.append(dev.malloc(32 << L))
+    dev.sync()
+    log("batched: inputs ready in %.1fs (MSM logs %s, NTT logs %s)" % (time.time() - t0, msm_logs, ntt_logs))
+
+    def one_at_a_time():
+        for j, L in enumerate(msm_logs):
+            msm_call(lab, d_b, d_k[j], r_plain[j], L)
+            dev.sync()
+        for j, L in enumerate(ntt_logs):
+            ntt_call(lab, x_in[j], y_plain[j], L)
+            dev.sync()
+
+    one_at_a_time()  # warm-up (pool growth, twiddles)
+    plain_s = []
+    for _ in range(reps):
+        t = time.perf_counter()
+        one_at_a_time()
+        plain_s.append(time.perf_counter() - t)
+    plain_s = float(np.median(plain_s))
+
+    msm_jobs = [B.msm_job(d_b, d_k[j], r_batch[j], L) for j, L in enumerate(msm_logs)]
+    ntt_jobs = [B.ntt_job(x_in[j], y_batch[j], L) for j, L in enumerate(ntt_logs)]
+    marr, narr = B.array_of(B.bc_amd_msm_job, msm_jobs), B.array_of(B.bc_amd_ntt_job, ntt_jobs)
+
+    def batch_calls():
+        B.call("bc_amd_msm_batch_async", marr, ctypes.c_uint(len(msm_jobs)), dev.pool, dev.stream)
+        B.call("bc_amd_ntt_batch_async", narr, ctypes.c_uint(len(ntt_jobs)), dev.pool, dev.stream)
+
+    batch_calls()
+    dev.sync()
+    batch_ms = lab.time_ms(batch_calls, reps)
+    plan = B.bc_amd_plan()
+    t = time.perf_counter()
+    B.call("bc_amd_plan_create", ctypes.byref(plan), marr, ctypes.c_uint(len(msm_jobs)), narr, ctypes.c_uint(len(ntt_jobs)))
+    plan_create_s = time.perf_counter() - t
+    ws = ctypes.c_size_t()
+    B.call("bc_amd_plan_workspace_bytes", plan, ctypes.byref(ws))
+    B.call("bc_amd_plan_launch", plan, dev.stream)
+    dev.sync()
+    plan_ms = lab.time_ms(lambda: B.call("bc_amd_plan_launch", plan, dev.stream), reps)
+    t = time.perf_counter()
+    for _ in range(reps):
+        B.call("bc_amd_plan_launch", plan, dev.stream)
+    dev.sync()
+    plan_wall_s = (time.perf_counter() - t) / reps
+    B.call("bc_amd_plan_destroy", plan)
+
+    # parity: NTT bytes identical to the plain calls, MSM folds identical and equal to the closed form
+    ntt_ok = all(np.array_equal(dev.download(y_plain[j], (1 << L, 8)), dev.download(y_batch[j], (1 << L, 8))) for j, L in enumerate(ntt_logs))
+    f_plain, f_batch = lab.folds(r_plain), lab.folds(r_batch)
+    exp = lab.expected_points(d_s, d_k, [1 << L for L in msm_logs])
+    msm_ok = bool(np.array_equal(f_plain, f_batch) and np.array_equal(f_batch, exp))
+    for p in [d_b, d_s, x_pool] + d_k + r_plain + r_batch + y_plain + y_batch:
+        dev.free(p)
+    jobs = len(msm_logs) + len(ntt_logs)
+    return {
+        "workload": "64 MSMs 2^12..2^20 (shared 2^20 bases) + 32 forward NTTs 2^10..2^22",
+        "jobs": jobs,
+        "one_at_a_time_jobs_per_s": round(jobs / plain_s, 1),
+        "one_at_a_time_ms": round(plain_s * 1e3, 2),
+        "batch_calls_ms": round(batch_ms, 2),
+        "plan_ms": round(plan_ms, 2),
+        "plan_wall_ms": round(plan_wall_s * 1e3, 2),
+        "plan_jobs_per_s": round(jobs / plan_wall_s, 1),
+        "speedup_vs_one_at_a_time": round(plain_s / plan_wall_s, 2),
+        "plan_create_s": round(plan_create_s, 2),
+        "plan_workspace_gib": round(ws.value / 2**30, 2),
+        "parity": {"ntt_bytes_equal_plain": bool(ntt_ok), "msm_folds_equal_plain_and_closed_form": msm_ok},
+    }
+
+
+# ---------------------------------------------------------------- served mix
+class ServedJobs:
+    def __init__(self, lab, n_jobs, seed=0xBC000005):
+        dev = lab.dev
+        self.lab = lab
+        rng = np.random.RandomState(seed & 0xFFFFFFFF)
+        t0 = time.time()
+        self.d_b, self.d_s = lab.bases(seed, 1 << 22)
+        self.k_pool = lab.upload_mont(msm_scalars_canonical(seed + 1, (1 << 22) + (1 << 21)))
+        self.x_pool = lab.upload_mont(uniform_fr(seed + 2, 1 << 23))
+        self.pn_idx = [dev.upload(pn_indexes(seed + 3 + t, 4, 20)) for t in range(2)]
+        self.pn_ks = lab.upload_mont(uniform_fr(seed + 5, 4))
+        self.jobs = []  # (kind, params, output ptr)
+        for j in range(n_jobs):
+            u = rng.randint(100)
+            if u < 50:
+                L = int(rng.randint(14, 23))
+                off = int(rng.randint(0, 1 << 21))
+                self.jobs.append(("msm", (L, self.k_pool + 32 * off, off), dev.malloc(254 * 96)))
+            elif u < 85:
+                L = int(rng.randint(12, 23))
+                off = int(rng.randint(0, (1 << 23) - (1 << L) + 1))
+                self.jobs.append(("ntt", (L, self.x_pool + 32 * off), dev.malloc(32 << L)))
+            else:
+                self.jobs.append(("pn", (self.pn_idx[j % 2], 4, 20), dev.malloc(32 * (4 << 20))))
+        dev.sync()
+        log("served: %d jobs ready in %.1fs (%d MSM, %d NTT, %d PN)" % (n_jobs, time.time() - t0, *[sum(1 for k in self.jobs if k[0] == t) for t in
+                                                                                                       ("msm", "ntt", "pn")]))
+
+    def plain(self, j):
+        kind, prm, out = self.jobs[j]
+        if kind == "msm":
+            msm_call(self.lab, self.d_b, prm[1], out, prm[0])
+        elif kind == "ntt":
+            ntt_call(self.lab, prm[1], out, prm[0])
+        else:
+            pn_call(self.lab, prm[0], self.pn_ks, out, prm[1], prm[2])
+
+    def job(self, j):
+        kind, prm, out = self.jobs[j]
+        job = B.bc_amd_job()
+        if kind == "msm":
+            job.kind = B.BC_AMD_JOB_MSM
+            job.msm = B.msm_job(self.d_b, prm[1], out, prm[0])
+        elif kind == "ntt":
+            job.kind = B.BC_AMD_JOB_NTT
+            job.ntt = B.ntt_job(prm[1], out, prm[0])
+        else:
+            job.kind = B.BC_AMD_JOB_PN
+            job.pn = B.bc_amd_pn_job(prm[0], self.pn_ks, out, prm[1], prm[2])
+        return job
+
+    def clear(self):
+        for kind, prm, out in self.jobs:
+            n = 254 * 96 if kind == "msm" else (32 << prm[0] if kind == "ntt" else 32 * (4 << 20))
+            B.call("bc_memset", ctypes.c_void_p(out), ctypes.c_int(0), ctypes.c_size_t(n))
+
+    def fingerprints(self):
+        """NTT/PN: digest of the output bytes; MSM: the folded affine point."""
+        dev = self.lab.dev
+        msm_idx = [j for j, (k, _, _) in enumerate(self.jobs) if k == "msm"]
+        folds = self.lab.folds([self.jobs[j][2] for j in msm_idx]) if msm_idx else []
+        fp = {}
+        for j, f in zip(msm_idx, folds):
+            fp[j] = f.tobytes()
+        for j, (kind, prm, out) in enumerate(self.jobs):
+            if kind == "ntt":
+                fp[j] = hashlib.sha1(dev.download(out, (1 << prm[0], 8)).tobytes()).hexdigest()
+            elif kind == "pn":
+                fp[j] = hashlib.sha1(dev.download(out, (4 << 20, 8)).tobytes()).hexdigest()
+        return fp
+
+    def msm_closed_form_ok(self, fp):
+        msm_idx = [j for j, (k, _, _) in enumerate(self.jobs) if k == "msm"]
+        if not msm_idx:
+            return True
+        # s prefix [0, n) pairs with the scalar slice [off, off + n)
+        exp = self.lab.expected_points(self.d_s, [self.jobs[j][1][1] for j in msm_idx], [1 << self.jobs[j][1][0] for j in msm_idx])
+        return all(fp[j] == e.tobytes() for j, e in zip(msm_idx, exp))
+
+    def free(self):
+        dev = self.lab.dev
+        for p in [self.d_b, self.d_s, self.k_pool, self.x_pool, self.pn_ks] + self.pn_idx + [o for _, _, o in self.jobs]:
+            dev.free(p)
+
+
+def run_served(srv, sj, arrivals=None):
+    """Submits every job (at `arrivals` seconds from start, or at once); returns stats per job."""
+    n = len(sj.jobs)
+    tickets = [None] * n
+    stats = [None] * n
+    jobs = [sj.job(j) for j in range(n)]
+
+    def waiter():
+        for j in range(n):
+            while tickets[j] is None:
+                time.sleep(50e-6)
+            stats[j] = srv.wait(tickets[j])
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for j in range(n):
+        if arrivals is not None:
+            delay = t0 + arrivals[j] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+        tickets[j] = srv.submit(jobs[j])
+    th.join()
+    return stats
+
+
+def served(lab, n_jobs, budget_gib, slots):
+    dev = lab.dev
+    sj = ServedJobs(lab, n_jobs)
+    n = len(sj.jobs)
+    # one at a time through the plain C ABI (sync after each), also the reference outputs
+    for j in range(min(n, 8)):
+        sj.plain(j)
+        dev.sync()
+    sj.clear()
+    t = time.perf_counter()
+    plain_done = []
+    for j in range(n):
+        sj.plain(j)
+        dev.sync()
+        plain_done.append(time.perf_counter() - t)
+    plain_s = time.perf_counter() - t
+    fp_plain = sj.fingerprints()
+    closed_ok = sj.msm_closed_form_ok(fp_plain)
+
+    srv = B.Server(hbm_budget_bytes=int(budget_gib * 2**30), max_in_flight=slots)
+    try:
+        run_served(srv, sj)  # warm-up (pool growth)
+        sj.clear()
+        st = run_served(srv, sj)
+        cap_s = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
+        fp_served = sj.fingerprints()
+        mismatches = [j for j in range(n) if fp_served[j] != fp_plain[j]]
+        capacity = n / cap_s
+        loads = {}
+        rng = np.random.RandomState(0xBC000006)
+        trace07 = None
+        for frac in (0.5, 0.7, 0.9):
+            lam = frac * capacity
+            arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
+            if frac == 0.7:
+                trace07 = arrivals
+            st = run_served(srv, sj, arrivals)
+            lat = [(s.done_ns - s.submit_ns) / 1e6 for s in st]
+            span = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
+            loads["%.1f" % frac] = {"rate_jobs_per_s": round(lam, 1), "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+                                    "jobs_per_s": round(n / span, 1), "mean_batch": round(float(np.mean([s.batch_jobs for s in st])), 2)}
+            log("served at %.1f x capacity: %s" % (frac, loads["%.1f" % frac]))
+    finally:
+        srv.close()
+    # the one-at-a-time FIFO under the same 0.7 x capacity arrivals
+    lat_plain = []
+    t0 = time.perf_counter()
+    for j in range(n):
+        now = time.perf_counter() - t0
+        if now < trace07[j]:
+            time.sleep(trace07[j] - now)
+        sj.plain(j)
+        dev.sync()
+        lat_plain.append((time.perf_counter() - t0 - trace07[j]) * 1e3)
+    sj.free()
+    return {
+        "workload": "Poisson mix: 50% MSM 2^14..2^22, 35% NTT 2^12..2^22, 15% PN 4x2^20 (random cycles 1..64)",
+        "jobs": n,
+        "server": {"hbm_budget_gib": budget_gib, "slots": slots},
+        "one_at_a_time_jobs_per_s": round(n / plain_s, 1),
+        "served_capacity_jobs_per_s": round(capacity, 1),
+        "speedup_vs_one_at_a_time": round(capacity / (n / plain_s), 2),
+        "latency_under_poisson": loads,
+        "one_at_a_time_fifo_at_0.7": {"p50_ms": round(pct(lat_plain, 50), 2), "p99_ms": round(pct(lat_plain, 99), 2)},
+        "parity": {"served_equal_plain": not mismatches, "mismatched_jobs": mismatches[:8], "msm_closed_form": bool(closed_ok)},
+    }
+
+
+# ---------------------------------------------------------------- CPU baseline (the oracle, host cores)
+def cpu_baseline(L_ntt_sample=20, L_msm_sample=16):
+    from oracle import oracle as O
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     x = O.random_fr(1, 1 << L_ntt_sample)
     t0 = time.time()
     O.ntt(x, L_ntt_sample)
@@ -130,19 +569,31 @@ def cpu_baseline(O, L_ntt_sample=20, L_msm_sample=16):
     t0 = time.time()
     O.msm(bases, k, L_msm_sample)
     t_msm = time.time() - t0
-    c16, c24 = O.msm_window_bits(L_msm_sample), O.msm_window_bits(24)
-    w16, w24 = (253 // c16) + 1, (253 // c24) + 1
-    # scale by the dominant work: n log n butterflies for the NTT, n * W mixed additions for the MSM
+    c_s, c_24 = O.msm_window_bits(L_msm_sample), O.msm_window_bits(MSM_METRIC_LOG)
+    w_s, w_24 = (253 // c_s) + 1, (253 // c_24) + 1
+    # scaled by the dominant work: n log n butterflies (NTT), n * W mixed additions (MSM)
     ntt24 = t_ntt * (24 * 2**24) / (L_ntt_sample * 2**L_ntt_sample)
-    msm24 = t_msm * (2**24 * w24) / (2**L_msm_sample * w16)
+    msm24 = t_msm * (2**24 * w_24) / (2**L_msm_sample * w_s)
     return {
         "value": round((ntt24 + msm24) * 1000.0, 1),
-        "unit": "ms per step (MSM 2^24 + NTT 2^24), extrapolated",
-        "cores": os.cpu_count(),
+        "unit": "ms per step (MSM 2^24 + NTT 2^24), extrapolated from the sample",
+        "cores": threads,
         "kind": "port",
-        "sample": "oracle NTT 2^%d on 1 thread (%.2fs) and MSM 2^%d with OpenMP over windows (%.2fs), scaled by n*log n and n*W"
-        % (L_ntt_sample, t_ntt, L_msm_sample, t_msm),
+        "sample": "C oracle (host_math restatement): NTT 2^%d on 1 thread %.2fs, MSM 2^%d with OpenMP over windows on %d threads %.2fs; "
+        "scaled by n log n and n W" % (L_ntt_sample, t_ntt, L_msm_sample, threads, t_msm),
     }
+
+
+def pmc_traffic():
+    """HBM bytes per ntt_pass_kernel launch from the committed rocprofv3 counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "ntt_pass_kernel_2e24_counters.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -150,53 +601,79 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--log-msm", type=int, default=24)
+    ap.add_argument("--log-msm", type=int, default=MSM_METRIC_LOG)
     ap.add_argument("--log-ntt", type=int, default=24)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--parts", default="tune,ntt,msm,batched,served,cpu",
+                    help="comma list of: tune, ntt, msm, batched, served, cpu, copy (a 2^24-element copy kernel, for counter scaling)")
+    ap.add_argument("--served-jobs", type=int, default=120)
+    ap.add_argument("--served-budget-gib", type=float, default=64.0)
+    ap.add_argument("--served-slots", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: the hot path is built and measured on a single MI355X (--gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the hot path is built and measured on a single MI355X; run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    from oracle import oracle as O
-
-    B.call("ff_set_up", ctypes.c_uint(25), ctypes.c_uint(14))
-    B.call("ntt_set_up")
-    B.call("msm_set_up")
-    dev = B.Device(0)
-    timer = Timer(dev)
+    parts = set(args.parts.split(","))
+    lab = Lab()
     log("device ready")
-    ntt_ms, lde_ms, ntt_ok = ntt_checked(dev, O, args.log_ntt, args.steps, args.warmup, timer)
-    log("ntt 2^%d: %.3f ms, lde x4: %.3f ms, round trip ok=%s" % (args.log_ntt, ntt_ms, lde_ms, ntt_ok))
-    msm_ms, msm_ok = msm_checked(dev, O, args.log_msm, args.steps, args.warmup, timer)
-    log("msm 2^%d: %.3f ms, closed form ok=%s" % (args.log_msm, msm_ms, msm_ok))
-    step_ms = msm_ms + ntt_ms
-    n_ntt = 1 << args.log_ntt
-    ntt_gbs = 64.0 * n_ntt / (ntt_ms * 1e-3) / 1e9
     out = {
         "metric": "ms per BN254 MSM/NTT at 2^24; batched jobs/s + p99 vs one-at-a-time calls",
-        "value": round(step_ms, 3),
-        "unit": "ms per step (one 2^24 G1 MSM + one 2^24 Fr NTT)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(step_ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32 limbs (exact Fq/Fr Montgomery arithmetic)",
-        "data": "synthetic, seeded",
-        "config": {"workload": "BN254 G1 MSM 2^%d + Fr NTT 2^%d (fwd n2b)" % (args.log_msm, args.log_ntt), "msm_log_n": args.log_msm,
-                   "ntt_log_n": args.log_ntt},
-        "msm_ms": round(msm_ms, 3),
-        "ntt_ms": round(ntt_ms, 3),
-        "lde_4_cosets_ms": round(lde_ms, 3),
-        "parity": {"msm_closed_form": msm_ok, "ntt_round_trip": ntt_ok},
-        "roofline": {"bound": "hbm", "kernel": "ntt_pass_kernel (whole NTT call)", "achieved": round(ntt_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ntt_gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "data": "synthetic, seeded (splitmix64)",
+        "config": {"workload": "BN254 G1 MSM 2^%d + Fr NTT 2^%d (forward, natural in, bit-reversed out)" % (args.log_msm, args.log_ntt),
+                   "msm_log_n": args.log_msm, "ntt_log_n": args.log_ntt},
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(O)
-    dev.close()
+    if "copy" in parts:
+        n = 1 << 24
+        src, dst = lab.dev.malloc(32 * n), lab.dev.malloc(32 * n)
+        for _ in range(3):
+            B.call("pn_distribute_values", ctypes.c_void_p(src), ctypes.c_void_p(dst), ctypes.c_uint(n), ctypes.c_uint(1), lab.dev.stream)
+        lab.dev.sync()
+        log("copy kernel: 3 launches of pn_distribute_values, 2^24 elements (512 MiB read + 512 MiB written each)")
+    if "tune" in parts:
+        out["tuning"] = autotune(lab)
+    if "ntt" in parts:
+        ntt_ms, inv_ms, lde_ms, ntt_ok = headline_ntt(lab, args.log_ntt, args.steps, args.warmup)
+        log("ntt 2^%d: %.3f ms, inverse %.3f ms, lde x4 %.3f ms, round trip ok=%s" % (args.log_ntt, ntt_ms, inv_ms, lde_ms, ntt_ok))
+        n = 1 << args.log_ntt
+        launches = (args.log_ntt + 11) // 12
+        per_launch_ms = ntt_ms / launches
+        achieved = 64.0 * n / (per_launch_ms * 1e-3) / 1e9
+        out.update({"ntt_ms": round(ntt_ms, 3), "ntt_inverse_ms": round(inv_ms, 3), "lde_4_cosets_ms": round(lde_ms, 3),
+                    "ntt_call_gbs": round(64.0 * n / (ntt_ms * 1e-3) / 1e9, 1)})
+        out["roofline"] = {"bound": "hbm", "kernel": "ntt_pass_kernel (2^%d, %d launches per NTT)" % (args.log_ntt, launches),
+                           "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                           "traffic": pmc_traffic(), "launch_ms": round(per_launch_ms, 4)}
+        out.setdefault("parity", {})["ntt_round_trip"] = ntt_ok
+    if "msm" in parts:
+        msm_ms, msm_ok, tuned_ms, tuned_ok = headline_msm(lab, args.log_msm, args.steps, args.warmup)
+        log("msm 2^%d: %.3f ms (closed form ok=%s), autotuned width %.3f ms (ok=%s)" % (args.log_msm, msm_ms, msm_ok, tuned_ms, tuned_ok))
+        out.update({"msm_ms": round(msm_ms, 3), "msm_autotuned_ms": round(tuned_ms, 3)})
+        W = 253 // 20 + 1 if args.log_msm == 24 else None
+        if W:
+            products = (1 << args.log_msm) * W * 10 * 136
+            tps = products / (msm_ms * 1e-3) / 1e12
+            out["roofline_msm"] = {"bound": "valu (v_mad_u64_u32 issue)", "kernel": "whole msm_execute_async call", "achieved": round(tps, 2),
+                                   "peak": VALU_MAD_PEAK_T, "unit": "T 32x32-bit products/s", "frac": round(tps / VALU_MAD_PEAK_T, 4)}
+        out.setdefault("parity", {}).update({"msm_closed_form": msm_ok, "msm_autotuned_closed_form": tuned_ok})
+    if "ntt" in parts and "msm" in parts:
+        step = out["msm_ms"] + out["ntt_ms"]
+        out.update({"value": round(step, 3), "unit": "ms per step (one 2^24 G1 MSM + one 2^24 Fr NTT)", "ms_per_step": round(step, 3)})
+    if "batched" in parts:
+        out["batched"] = batched(lab, max(1, args.steps))
+        log("batched: %s" % out["batched"])
+    if "served" in parts:
+        out["served"] = served(lab, args.served_jobs, args.served_budget_gib, args.served_slots)
+        log("served: %s" % out["served"])
+    if "cpu" in parts:
+        out["cpu_baseline"] = cpu_baseline()
+        log("cpu baseline: %s" % out["cpu_baseline"])
+    lab.dev.close()
     print(json.dumps(out), flush=True)
 
 

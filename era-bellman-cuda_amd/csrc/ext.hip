@@ -4,6 +4,10 @@
 // T[j][v] = v * 2^(8j) * G for 32 byte positions and v in [1, 256); a scalar
 // costs at most 32 mixed additions plus one field inversion for the affine
 // output.  The table is built once per device.
+//
+// bc_amd_msm_fold: the caller-side fold of the 254 per-bit MSM results
+// (sum = 2*sum + results[i], src/bellman-cuda.h:496-505) on the device, one
+// thread per MSM, with the affine normalisation.
 #include <mutex>
 
 #include "common.h"
@@ -63,6 +67,32 @@ __global__ __launch_bounds__(128) void generator_multiples_kernel(const fe *scal
   store_fe(&out[i].y, a.y);
 }
 
+constexpr unsigned FOLD_MAX = 64; // MSMs per fold launch
+
+struct fold_args {
+  const g1_jacobian *results[FOLD_MAX];
+};
+
+// Horner over the result bits in XYZZ coordinates: 253 doublings and 254
+// additions per MSM, then one inversion.
+__global__ void msm_fold_kernel(const fold_args a, g1_affine *out, unsigned count) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count)
+    return;
+  const g1_jacobian *r = a.results[j];
+  g1_xyzz acc = xyzz_infinity();
+  for (int i = 253; i >= 0; i--) {
+    acc = xyzz_dbl(acc);
+    const fe z = load_fe(&r[i].z);
+    const fe zz = mul_lazy<Fq>(z, z);
+    const g1_xyzz t = {load_fe(&r[i].x), load_fe(&r[i].y), zz, mul_lazy<Fq>(zz, z)};
+    acc = xyzz_add(acc, t);
+  }
+  const g1_affine p = xyzz_to_affine(acc);
+  store_fe(&out[j].x, p.x);
+  store_fe(&out[j].y, p.y);
+}
+
 hipError_t comb_table(g1_affine **table) {
   std::lock_guard<std::mutex> lock(g_comb_mutex);
   int d = 0;
@@ -110,6 +140,23 @@ bc_error bc_amd_generator_multiples(const void *scalars, void *bases, unsigned c
   generator_multiples_kernel<<<grid_for(count, 128), 128, 0, to_stream(stream)>>>(static_cast<const fe *>(scalars), table, static_cast<g1_affine *>(bases),
                                                                                   count);
   return as_bc(hipGetLastError());
+}
+
+bc_error bc_amd_msm_fold(const void *const *results, void *points, unsigned count, bc_stream stream) {
+  if (count && (!results || !points))
+    return bc_error_invalid_value;
+  for (unsigned base = 0; base < count; base += FOLD_MAX) {
+    const unsigned k = count - base < FOLD_MAX ? count - base : FOLD_MAX;
+    fold_args a{};
+    for (unsigned j = 0; j < k; j++) {
+      if (!results[base + j])
+        return bc_error_invalid_value;
+      a.results[j] = static_cast<const g1_jacobian *>(results[base + j]);
+    }
+    msm_fold_kernel<<<1, FOLD_MAX, 0, to_stream(stream)>>>(a, static_cast<g1_affine *>(points) + base, k);
+    BC_TRY_BC(hipGetLastError());
+  }
+  return bc_success;
 }
 
 } // extern "C"

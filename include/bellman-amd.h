@@ -21,6 +21,14 @@ extern "C" {
  * Requires ff_set_up.  A zero scalar yields the affine encoding (0, 0). */
 bc_error bc_amd_generator_multiples(const void *scalars, void *bases, unsigned count, bc_stream stream);
 
+/* points[j] = the fold of the 254 per-bit results of MSM j
+ * (sum = 2*sum + results[j][i] for i = 253..0) as a canonical affine
+ * Montgomery point, (0, 0) for infinity.  `results` is a host array of
+ * `count` device pointers (254 Jacobian points each); `points` is device
+ * memory.  One thread per MSM: the 253 dependent doublings take about a
+ * millisecond whatever the count. */
+bc_error bc_amd_msm_fold(const void *const *results, void *points, unsigned count, bc_stream stream);
+
 /* ------------------------------------------------------------------------ */
 /* Jobs.  All buffers are device memory.                                     */
 

@@ -341,3 +341,29 @@ def test_autotune_sets_valid_tuning_and_keeps_results(bellman, oracle, shared_ba
     finally:
         B.call("bc_amd_tuning_load", saved.encode())
         dev.free(dk)
+
+
+def test_msm_fold_on_device(bellman, oracle, shared_bases):
+    """bc_amd_msm_fold equals the host fold + affine normalisation, infinity as (0, 0)."""
+    _, dev = bellman
+    db, bases, logs = shared_bases
+    ptrs, exp = [], []
+    for j, L in enumerate([0, 4, 9, 12, 3]):
+        k = oracle.msm_scalars(0xBC00004F + j, 1 << L) if j != 4 else oracle.to_mont([0] * 8)
+        dk, dr = dev.upload(k), dev.malloc(254 * 96)
+        msm_batch(dev, [B.msm_job(db, dk, dr, L)])
+        res = dev.download(dr, (254, 24))
+        folded = oracle.to_affine(oracle.msm_fold(res))[0]
+        assert np.array_equal(folded, closed_form(oracle, logs[: 1 << L], k))
+        exp.append(folded if j != 4 else np.zeros(16, dtype=np.uint32))
+        ptrs.append(dr)
+        dev.free(dk)
+    d_p = dev.malloc(64 * len(ptrs))
+    B.call("bc_amd_msm_fold", (ctypes.c_void_p * len(ptrs))(*ptrs), ctypes.c_void_p(d_p), ctypes.c_uint(len(ptrs)), dev.stream)
+    dev.sync()
+    got = dev.download(d_p, (len(ptrs), 16))
+    for j in range(len(ptrs)):
+        assert np.array_equal(got[j], exp[j]), j
+    dev.free(d_p)
+    for p in ptrs:
+        dev.free(p)
