@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libbellman.so")
+# BELLMAN_AMD_LIB points the binding at another build of the library (e.g. a variant under test).
+LIB_PATH = os.environ.get("BELLMAN_AMD_LIB", os.path.join(PKG_DIR, "libbellman.so"))
 
 bc_success = 0
 bc_error_invalid_value = 1

@@ -34,7 +34,7 @@ BC_HD g1_affine xyzz_to_affine(const g1_xyzz &p) {
   return {mul<Fq>(p.x, izz), mul<Fq>(p.y, izzz)};
 }
 
-__global__ void comb_table_kernel(const g1_affine *base_per_byte, g1_affine *table) {
+__global__ __launch_bounds__(128) void comb_table_kernel(const g1_affine *base_per_byte, g1_affine *table) {
   const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= COMB_BYTES * COMB_ENTRIES)
     return;
@@ -75,7 +75,7 @@ struct fold_args {
 
 // Horner over the result bits in XYZZ coordinates: 253 doublings and 254
 // additions per MSM, then one inversion.
-__global__ void msm_fold_kernel(const fold_args a, g1_affine *out, unsigned count) {
+__global__ __launch_bounds__(FOLD_MAX) void msm_fold_kernel(const fold_args a, g1_affine *out, unsigned count) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count)
     return;

@@ -181,56 +181,9 @@ template <class F> BC_HD fe neg2(const fe &a) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// acc += a*b with the 64-bit overflow counted in c2 (v_mad_u64_u32 carry-out -> v_addc_co_u32)
-__device__ __forceinline__ void mac_vv(uint64_t &acc, uint32_t &c2, uint32_t a, uint32_t b) {
-  asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc" : "+v"(acc), "+v"(c2) : "v"(a), "v"(b) : "vcc");
-}
-
-// same with a uniform (SGPR) second operand, e.g. a modulus limb
-__device__ __forceinline__ void mac_vs(uint64_t &acc, uint32_t &c2, uint32_t a, uint32_t b) {
-  asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc" : "+v"(acc), "+v"(c2) : "v"(a), "s"(b) : "vcc");
-}
-
-// Montgomery product by finely integrated product scanning: column k of
-// a*b + m*N accumulates in a 64-bit register pair (v_mad_u64_u32) with its
-// overflow in c2; m_k zeroes the low word of columns 0..7, columns 8..15 are
-// the result.  The first product of a column cannot overflow (the carried-in
-// value is < 2^37), so it skips the carry.  Inputs < 2m, output < 2m.
-template <class F> __device__ __forceinline__ fe mul_lazy_device(const fe &a, const fe &b) {
-  uint32_t m[8];
-  fe r;
-  uint64_t acc = 0;
-  uint32_t c2 = 0;
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    acc += (uint64_t)a.v[0] * b.v[k];
-#pragma unroll
-    for (int i = 1; i <= k; i++)
-      mac_vv(acc, c2, a.v[i], b.v[k - i]);
-#pragma unroll
-    for (int i = 0; i < k; i++)
-      mac_vs(acc, c2, m[i], F::m[k - i]);
-    m[k] = (uint32_t)acc * F::inv;
-    mac_vs(acc, c2, m[k], F::m[0]);
-    acc = (acc >> 32) | ((uint64_t)c2 << 32);
-    c2 = 0;
-  }
-#pragma unroll
-  for (int k = 8; k < 15; k++) {
-    acc += (uint64_t)a.v[k - 7] * b.v[7];
-#pragma unroll
-    for (int i = k - 6; i <= 7; i++)
-      mac_vv(acc, c2, a.v[i], b.v[k - i]);
-#pragma unroll
-    for (int i = k - 7; i <= 7; i++)
-      mac_vs(acc, c2, m[i], F::m[k - i]);
-    r.v[k - 8] = (uint32_t)acc;
-    acc = (acc >> 32) | ((uint64_t)c2 << 32);
-    c2 = 0;
-  }
-  r.v[7] = (uint32_t)acc;
-  return r;
-}
+// Montgomery product a*b/R by finely integrated product scanning, one asm
+// statement per column (generated; see gen_mont_asm.py).  Inputs < 2m, output < 2m.
+#include "mont_asm.h"
 #endif
 
 // Montgomery product a*b/R; inputs < 2m, output < 2m.

@@ -15,12 +15,14 @@
 //
 // Pipeline:
 //   1. digits:     one thread per scalar writes, for each window, the key
-//                  (batch-global bucket id, or NO_BUCKET for a zero digit) and
-//                  the value sign | point index;
-//   2. sort:       one stable LSD radix sort of all (key, value) pairs of the
-//                  batch (8-bit digits, per-tile LDS histograms, wave-ballot
-//                  ranking: no global atomics, so heavy buckets cost nothing
-//                  extra), then run detection gives each bucket's [start, end);
+//                  (the window-local bucket d - 1, or 2^(c-1) for a zero
+//                  digit) and the value sign | point index;
+//   2. sort:       one stable segmented radix sort of all (key, value) pairs
+//                  of the batch, each window a segment (c key bits: two passes
+//                  of <= 11 bits up to c = 22; per-tile LDS histograms,
+//                  wave-ballot ranking, no global atomics, so heavy buckets
+//                  cost nothing extra), then run detection gives each bucket's
+//                  [start, end);
 //   3. accumulate: buckets are cut into pieces of at most `piece` points; one
 //                  thread accumulates one piece with XYZZ mixed additions, so a
 //                  heavy bucket (e.g. digit 1 of window 0 when many scalars are
@@ -72,7 +74,6 @@ unsigned msm_window_bits(unsigned log_n) {
 namespace {
 
 constexpr uint32_t SIGN = 0x80000000u;
-constexpr uint32_t NO_BUCKET = 0x7fffffffu; // every key bit below 31 set: sorts after all buckets
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
 constexpr unsigned SPLIT_TERMS = 16;
@@ -90,6 +91,8 @@ struct job_desc {
   uint32_t top_begin;    // first top-digit slot
   uint32_t piece;        // most points per accumulating thread
 };
+
+static_assert(MSM_MAX_JOBS <= SEG_MAX_GROUPS, "one sort segment group per job");
 
 struct job_table {
   uint32_t count;
@@ -116,6 +119,9 @@ struct by_scalar {
 struct by_bucket {
   __device__ uint32_t operator()(const job_desc &d) const { return d.bucket_begin; }
 };
+struct by_entry {
+  __device__ uint32_t operator()(const job_desc &d) const { return (uint32_t)d.entry_begin; }
+};
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,8 +136,8 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
   return hipGetLastError();
 }
 
-// 1. signed digits.  Entry (w, i) of a job gets key = bucket_begin + w*2^(c-1) + d - 1
-// (NO_BUCKET for d = 0) and value = sign << 31 | i.
+// 1. signed digits.  Entry (w, i) of a job gets the window-local key d - 1
+// (2^(c-1), after every bucket, for d = 0) and value = sign << 31 | i.
 __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint32_t total, uint32_t *__restrict__ keys, uint32_t *__restrict__ values) {
   using F = FrParams;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,24 +162,30 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
       sign ^= SIGN;
     }
     const uint64_t e = jd.entry_begin + (uint64_t)w * n + i;
-    keys[e] = d ? jd.bucket_begin + w * top + (d - 1) : NO_BUCKET;
+    keys[e] = d ? d - 1 : top;
     values[e] = sign | i;
   }
 }
 
 // 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted arrays
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const uint32_t *__restrict__ keys, size_t entries, uint32_t *__restrict__ start,
-                                                                uint32_t *__restrict__ end) {
-  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// (window w of a job is the segment [entry_begin + w n, entry_begin + (w+1) n))
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint32_t *__restrict__ keys, uint32_t entries,
+                                                                uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= entries)
     return;
+  const job_desc &jd = jobs.job[find_job(jobs, e, by_entry())];
   const uint32_t k = keys[e];
-  if (k == NO_BUCKET)
-    return;
-  if (e == 0 || keys[e - 1] != k)
-    start[k] = (uint32_t)e;
-  if (e + 1 == entries || keys[e + 1] != k)
-    end[k] = (uint32_t)(e + 1);
+  const uint32_t top = 1u << (jd.c - 1);
+  if (k == top)
+    return; // zero digits
+  const uint32_t local = e - (uint32_t)jd.entry_begin, w = local / jd.n;
+  const uint32_t seg_begin = (uint32_t)jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
+  const uint32_t b = jd.bucket_begin + w * top + k;
+  if (e == seg_begin || keys[e - 1] != k)
+    start[b] = e;
+  if (e + 1 == seg_end || keys[e + 1] != k)
+    end[b] = e + 1;
 }
 
 // counts[b] = end[b] - start[b] (in place over `end`), pieces of at most `piece` points
@@ -215,12 +227,19 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   const uint32_t stop = offsets[b] + counts[b];
   const uint32_t end = begin + piece < stop ? begin + piece : stop;
   g1_xyzz acc = xyzz_infinity();
-  for (uint32_t e = begin; e < end; e++) {
-    const uint32_t ref = sorted[e];
+  if (begin < end) {
+    // the next point's loads are issued before the current addition
+    uint32_t ref = sorted[begin];
     g1_affine pt = load_affine(bases + (ref & ~SIGN));
-    if (ref & SIGN)
-      pt.y = neg2<Fq>(pt.y);
-    acc = xyzz_add_affine(acc, pt);
+    for (uint32_t e = begin; e < end; e++) {
+      const uint32_t ref_next = e + 1 < end ? sorted[e + 1] : ref;
+      const g1_affine next = load_affine(bases + (ref_next & ~SIGN));
+      if (ref & SIGN)
+        pt.y = neg2<Fq>(pt.y);
+      acc = xyzz_add_affine(acc, pt);
+      pt = next;
+      ref = ref_next;
+    }
   }
   store_xyzz(partials + p, acc);
 }
@@ -483,7 +502,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     max_np = std::max(max_np, (d.n + d.piece - 1) / d.piece);
     max_pieces += ((uint64_t)d.W * d.n + d.piece - 1) / d.piece + ((uint64_t)d.W << (c - 1)); // sum of ceil(count/piece) over buckets
   }
-  if (entries >= (1ull << 31) || nb64 >= NO_BUCKET)
+  if (entries >= (1ull << 31) || nb64 >= (1ull << 31))
     return hipErrorInvalidValue;
   const uint32_t nb = (uint32_t)nb64;
 
@@ -499,13 +518,16 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     BC_TRY(ar.get(&sorted_keys, entries));
     digits_kernel<<<grid_for(scalars_total, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, scalars_total, keys, vals);
     BC_LAUNCH_CHECK();
-    unsigned key_bits = 0;
-    while ((1ull << key_bits) <= nb)
-      key_bits++;
-    BC_TRY(radix_sort_pairs(keys, sorted_keys, vals, sorted, entries, 0, key_bits, ar));
+    seg_group groups[MSM_MAX_JOBS];
+    unsigned key_bits = 0; // local keys are <= 2^(c-1)
+    for (unsigned j = 0; j < count; j++) {
+      groups[j] = {jt.job[j].entry_begin, jt.job[j].n, jt.job[j].W};
+      key_bits = std::max(key_bits, jt.job[j].c);
+    }
+    BC_TRY(segmented_sort_pairs(keys, sorted_keys, vals, sorted, groups, count, key_bits, ar));
     BC_TRY(fill_u32(offsets, nb, 0, s));
     BC_TRY(fill_u32(counts, nb, 0, s));
-    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(sorted_keys, entries, offsets, counts);
+    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, sorted_keys, (uint32_t)entries, offsets, counts);
     BC_LAUNCH_CHECK();
     BC_TRY(ar.put(keys));
     BC_TRY(ar.put(vals));

@@ -13,3 +13,22 @@ hipError_t radix_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const u
                             unsigned end_bit, arena &ar);
 
 } // namespace bc
+
+namespace bc {
+
+// Segments of a segmented sort, in groups: group g holds `count` consecutive
+// segments of `length` entries each, starting at entry `begin`; groups are
+// contiguous and in order.  Keys are sorted within each segment (stably);
+// entries never leave their segment.
+constexpr unsigned SEG_MAX_GROUPS = 64;
+
+struct seg_group {
+  uint64_t begin;
+  uint32_t length;
+  uint32_t count;
+};
+
+hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
+                                unsigned group_count, unsigned key_bits, arena &ar);
+
+} // namespace bc

@@ -1,5 +1,6 @@
-// Stable LSD radix sort, 8 key bits per pass, wave64 ranking.
+// Stable LSD radix sorts with wave64 ranking.
 //
+// radix_sort_*: 8 key bits per pass over one array.
 // Each pass: (1) per-tile digit histograms written digit-major,
 // (2) one exclusive scan over the [digit][tile] table (so tiles of the same
 // digit stay in input order), (3) a stable scatter: inside a tile, elements are
@@ -137,6 +138,223 @@ hipError_t radix_sort_impl(const uint32_t *keys_in, uint32_t *keys_out, const ui
 }
 
 } // namespace
+
+// ---------------------------------------------------------------- segmented
+//
+// segmented_sort_pairs: up to 11 key bits per pass; every tile (4096 entries)
+// lies inside one segment.  The histogram table is laid out
+// [segment][digit][tile of the segment], so one exclusive scan over the whole
+// table gives each (tile, digit) its output position and keeps every segment in
+// place (segments are contiguous and all earlier segments sort before).  The
+// scatter ranks a tile's entries in input order: each wave owns 1024
+// consecutive entries, ranks them 64 at a time by ballots over the digit bits
+// with a running per-wave digit count in LDS, and after one block barrier adds
+// the counts of the waves before it and the tile's global offset.
+
+namespace {
+
+constexpr unsigned SS_BLOCK = 256;
+constexpr unsigned SS_WAVES = SS_BLOCK / 64;
+constexpr unsigned SS_ITEMS = 16; // entries per thread
+constexpr unsigned SS_TILE = SS_BLOCK * SS_ITEMS;
+constexpr unsigned SS_MAX_BITS = 11;
+constexpr unsigned SS_MAX_DIGITS = 1u << SS_MAX_BITS;
+
+struct ss_group {
+  uint64_t begin;       // first entry
+  uint64_t table_begin; // first histogram slot
+  uint32_t length, count;
+  uint32_t tile_begin; // first tile
+  uint32_t tiles_per_seg;
+};
+
+struct ss_table {
+  uint32_t count;
+  ss_group g[SEG_MAX_GROUPS];
+};
+static_assert(sizeof(ss_table) <= 4096, "kernel argument budget");
+
+struct ss_tile {
+  uint64_t start;  // first entry of the tile
+  uint64_t slot0;  // histogram slot of digit 0; digit d at slot0 + d * stride
+  uint32_t len;    // entries in the tile
+  uint32_t stride; // tiles of the segment
+};
+
+__device__ __forceinline__ ss_tile ss_locate(const ss_table &t, unsigned digits) {
+  unsigned lo = 0, hi = t.count;
+  while (hi - lo > 1) {
+    const unsigned mid = (lo + hi) >> 1;
+    if (t.g[mid].tile_begin <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const ss_group &g = t.g[lo];
+  const uint32_t local = blockIdx.x - g.tile_begin;
+  const uint32_t seg = local / g.tiles_per_seg, k = local % g.tiles_per_seg;
+  ss_tile r;
+  r.start = g.begin + (uint64_t)seg * g.length + (uint64_t)k * SS_TILE;
+  const uint32_t rem = g.length - k * SS_TILE;
+  r.len = rem < SS_TILE ? rem : SS_TILE;
+  r.slot0 = g.table_begin + (uint64_t)seg * digits * g.tiles_per_seg + k;
+  r.stride = g.tiles_per_seg;
+  return r;
+}
+
+__global__ __launch_bounds__(SS_BLOCK) void ss_histogram_kernel(const ss_table t, const uint32_t *__restrict__ keys, unsigned shift, unsigned bits,
+                                                                 uint32_t *__restrict__ table) {
+  __shared__ uint32_t hist[SS_MAX_DIGITS];
+  const unsigned digits = 1u << bits;
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
+    hist[d] = 0;
+  __syncthreads();
+  const ss_tile tl = ss_locate(t, digits);
+  const uint32_t mask = digits - 1;
+  for (unsigned i = threadIdx.x; i < tl.len; i += SS_BLOCK)
+    atomicAdd(&hist[(keys[tl.start + i] >> shift) & mask], 1u);
+  __syncthreads();
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
+    table[tl.slot0 + (uint64_t)d * tl.stride] = hist[d];
+}
+
+template <bool WITH_VALUES>
+__global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kernel(const ss_table t, const uint32_t *__restrict__ keys_in, uint32_t *__restrict__ keys_out,
+                                                               const uint32_t *__restrict__ vals_in, uint32_t *__restrict__ vals_out, unsigned shift,
+                                                               unsigned bits, const uint32_t *__restrict__ table) {
+  __shared__ uint32_t wave_hist[SS_WAVES][SS_MAX_DIGITS];
+  const unsigned digits = 1u << bits;
+  const uint32_t mask = digits - 1;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
+    for (unsigned w = 0; w < SS_WAVES; w++)
+      wave_hist[w][d] = 0;
+  __syncthreads();
+  const ss_tile tl = ss_locate(t, digits);
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint32_t key[SS_ITEMS], val[SS_ITEMS], rank[SS_ITEMS];
+#pragma unroll
+  for (unsigned s = 0; s < SS_ITEMS; s++) {
+    const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
+    const bool valid = i < tl.len;
+    key[s] = valid ? keys_in[tl.start + i] : 0u;
+    val[s] = (WITH_VALUES && valid) ? vals_in[tl.start + i] : 0u;
+    const uint32_t d = (key[s] >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (unsigned b = 0; b < bits; b++) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const unsigned before = __popcll(peers & lt_mask);
+    // all lanes of the wave read the running count before its leader advances it
+    const uint32_t base = valid ? wave_hist[wave][d] : 0u;
+    rank[s] = base + before;
+    if (valid && before == 0)
+      wave_hist[wave][d] = base + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK) {
+    uint32_t run = table[tl.slot0 + (uint64_t)d * tl.stride];
+    for (unsigned w = 0; w < SS_WAVES; w++) {
+      const uint32_t c = wave_hist[w][d];
+      wave_hist[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (unsigned s = 0; s < SS_ITEMS; s++) {
+    const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
+    if (i < tl.len) {
+      const uint32_t pos = wave_hist[wave][(key[s] >> shift) & mask] + rank[s];
+      keys_out[pos] = key[s];
+      if (WITH_VALUES)
+        vals_out[pos] = val[s];
+    }
+  }
+}
+
+} // namespace
+
+hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
+                                unsigned group_count, unsigned key_bits, arena &ar) {
+  hipStream_t stream = ar.stream();
+  if (group_count == 0)
+    return hipSuccess;
+  if (group_count > SEG_MAX_GROUPS || key_bits > 32)
+    return hipErrorInvalidValue;
+  uint64_t n = 0, tiles = 0;
+  for (unsigned g = 0; g < group_count; g++) {
+    if (groups[g].begin != n || groups[g].length == 0)
+      return hipErrorInvalidValue; // contiguous from entry 0
+    n += (uint64_t)groups[g].length * groups[g].count;
+    tiles += (uint64_t)groups[g].count * ((groups[g].length + SS_TILE - 1) / SS_TILE);
+  }
+  if (n == 0)
+    return hipSuccess;
+  if (n >= (1ull << 32) || tiles >= (1ull << 31))
+    return hipErrorInvalidValue;
+  const bool with_values = values_in != nullptr;
+  if (key_bits == 0) {
+    BC_TRY(hipMemcpyAsync(keys_out, keys_in, n * 4, hipMemcpyDeviceToDevice, stream));
+    if (with_values)
+      BC_TRY(hipMemcpyAsync(values_out, values_in, n * 4, hipMemcpyDeviceToDevice, stream));
+    return hipSuccess;
+  }
+  const unsigned passes = (key_bits + SS_MAX_BITS - 1) / SS_MAX_BITS;
+  // balanced digit widths, wider first
+  unsigned widths[4], shifts[4];
+  for (unsigned p = 0, shift = 0; p < passes; p++) {
+    widths[p] = key_bits / passes + (p < key_bits % passes ? 1 : 0);
+    shifts[p] = shift;
+    shift += widths[p];
+  }
+  uint64_t table_entries = 0;
+  for (unsigned g = 0; g < group_count; g++)
+    table_entries += (uint64_t)groups[g].count * (1u << widths[0]) * ((groups[g].length + SS_TILE - 1) / SS_TILE);
+  uint32_t *table = nullptr, *tmp_k = nullptr, *tmp_v = nullptr;
+  BC_TRY(ar.get(&table, table_entries));
+  BC_TRY(ar.get(&tmp_k, n));
+  if (with_values)
+    BC_TRY(ar.get(&tmp_v, n));
+  const uint32_t *src_k = keys_in, *src_v = values_in;
+  for (unsigned p = 0; p < passes; p++) {
+    const unsigned digits = 1u << widths[p];
+    ss_table t{};
+    t.count = group_count;
+    uint64_t slots = 0;
+    uint32_t tile = 0;
+    for (unsigned g = 0; g < group_count; g++) {
+      ss_group &sg = t.g[g];
+      sg.begin = groups[g].begin;
+      sg.length = groups[g].length;
+      sg.count = groups[g].count;
+      sg.tiles_per_seg = (groups[g].length + SS_TILE - 1) / SS_TILE;
+      sg.tile_begin = tile;
+      sg.table_begin = slots;
+      tile += sg.count * sg.tiles_per_seg;
+      slots += (uint64_t)sg.count * digits * sg.tiles_per_seg;
+    }
+    const bool to_out = ((passes - 1 - p) % 2) == 0; // the last pass lands in *_out
+    uint32_t *dst_k = to_out ? keys_out : tmp_k;
+    uint32_t *dst_v = to_out ? values_out : tmp_v;
+    ss_histogram_kernel<<<tile, SS_BLOCK, 0, stream>>>(t, src_k, shifts[p], widths[p], table);
+    BC_LAUNCH_CHECK();
+    BC_TRY(scan_u32_exclusive(table, table, slots, ar));
+    if (with_values)
+      ss_scatter_kernel<true><<<tile, SS_BLOCK, 0, stream>>>(t, src_k, dst_k, src_v, dst_v, shifts[p], widths[p], table);
+    else
+      ss_scatter_kernel<false><<<tile, SS_BLOCK, 0, stream>>>(t, src_k, dst_k, nullptr, nullptr, shifts[p], widths[p], table);
+    BC_LAUNCH_CHECK();
+    src_k = dst_k;
+    src_v = dst_v;
+  }
+  BC_TRY(ar.put(table));
+  BC_TRY(ar.put(tmp_k));
+  if (with_values)
+    BC_TRY(ar.put(tmp_v));
+  return hipSuccess;
+}
 
 hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned begin_bit, unsigned end_bit, arena &ar) {
   return radix_sort_impl(keys_in, keys_out, nullptr, nullptr, n, begin_bit, end_bit, ar);
