@@ -555,7 +555,7 @@ def served(lab, n_jobs, budget_gib, slots):
 
 
 # ---------------------------------------------------------------- CPU baseline (the oracle, host cores)
-def cpu_baseline(L_ntt_sample=20, L_msm_sample=16):
+def cpu_baseline(L_ntt_sample=22, L_msm_sample=20):
     from oracle import oracle as O
 
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

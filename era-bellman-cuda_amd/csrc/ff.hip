@@ -450,7 +450,7 @@ bc_error ff_select(const void *source, void *destination, const unsigned *indexe
 
 bc_error ff_sort_u32(ff_sort_u32_configuration cfg) {
   arena ar = arena::pool_mode(to_pool(cfg.mem_pool), to_stream(cfg.stream));
-  return as_bc(radix_sort_keys(static_cast<const uint32_t *>(cfg.values), static_cast<uint32_t *>(cfg.sorted_values), cfg.count, 0, 32, ar));
+  return as_bc(radix_sort_keys(static_cast<const uint32_t *>(cfg.values), static_cast<uint32_t *>(cfg.sorted_values), cfg.count, 32, ar));
 }
 
 bc_error pn_set_values_from_packed_bits(void *values, const void *packet_bits, unsigned count, bc_stream stream) {

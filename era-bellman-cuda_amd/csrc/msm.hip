@@ -28,7 +28,8 @@
 //                  heavy bucket (e.g. digit 1 of window 0 when many scalars are
 //                  1) is spread over many threads;
 //   4. fix-up:     buckets of up to 8 pieces are summed by one thread; heavier
-//                  buckets are listed and combined by a log-depth tree;
+//                  buckets are listed and reduced by blocks (chunks of 2048
+//                  pieces: serial sums of 8, then an LDS tree);
 //   5. per-bit reduction: windows are split recursively into low/high halves
 //                  of their index bits until each holds one bit (sums split
 //                  across threads, then folded pairwise);
@@ -265,15 +266,23 @@ __global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partial
   store_xyzz(partials + off, acc);
 }
 
-// exclusive prefix of the heavy buckets' piece counts (one block, chunked)
+// Heavy buckets are reduced in chunks of HEAVY_CHUNK pieces, one block per
+// chunk: each thread sums HEAVY_PER_THREAD consecutive pieces, then a tree in
+// LDS; the chunk's sum overwrites the chunk's first piece.  A second kernel
+// adds a bucket's chunk sums in order into its first piece.
+constexpr unsigned HEAVY_BLOCK = 256;
+constexpr unsigned HEAVY_PER_THREAD = 8;
+constexpr unsigned HEAVY_CHUNK = HEAVY_BLOCK * HEAVY_PER_THREAD;
+
+// exclusive prefix of the heavy buckets' chunk counts (one block, chunked)
 __global__ __launch_bounds__(256) void heavy_prefix_kernel(const uint32_t *heavy, const uint32_t *heavy_count, const uint32_t *piece_count,
-                                                           uint32_t *heavy_offset, uint32_t *heavy_total) {
+                                                           uint32_t *chunk_offset, uint32_t *chunk_total) {
   __shared__ uint32_t sh[256];
   const uint32_t count = *heavy_count;
   uint32_t carry = 0;
   for (uint32_t base = 0; base < count; base += 256) {
     const uint32_t i = base + threadIdx.x;
-    const uint32_t v = i < count ? piece_count[heavy[i]] : 0;
+    const uint32_t v = i < count ? (piece_count[heavy[i]] + HEAVY_CHUNK - 1) / HEAVY_CHUNK : 0;
     sh[threadIdx.x] = v;
     __syncthreads();
     for (unsigned off = 1; off < 256; off <<= 1) {
@@ -283,34 +292,64 @@ __global__ __launch_bounds__(256) void heavy_prefix_kernel(const uint32_t *heavy
       __syncthreads();
     }
     if (i < count)
-      heavy_offset[i] = carry + sh[threadIdx.x] - v;
+      chunk_offset[i] = carry + sh[threadIdx.x] - v;
     carry += sh[255];
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    *heavy_total = carry;
+    *chunk_total = carry;
 }
 
-// 4b. one tree level over the pieces of heavy buckets: piece q absorbs q + stride when q % (2 stride) == 0
-__global__ __launch_bounds__(ACC_BLOCK) void heavy_tree_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
-                                                               const uint32_t *heavy_offset, const uint32_t *heavy_total, const uint32_t *piece_offset,
-                                                               const uint32_t *piece_count, unsigned stride) {
-  const uint32_t total = *heavy_total, count = *heavy_count;
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < total; h += gridDim.x * blockDim.x) {
-    uint32_t lo = 0, hi = count; // heavy bucket holding flattened piece h
+// 4b. one block per chunk (grid-stride over all chunks of all heavy buckets)
+__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+                                                                  const uint32_t *chunk_offset, const uint32_t *chunk_total,
+                                                                  const uint32_t *piece_offset, const uint32_t *piece_count) {
+  __shared__ g1_xyzz sh[HEAVY_BLOCK];
+  const uint32_t total = *chunk_total, count = *heavy_count;
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    uint32_t lo = 0, hi = count; // heavy bucket of chunk c
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) >> 1;
-      if (heavy_offset[mid] <= h)
+      if (chunk_offset[mid] <= c)
         lo = mid;
       else
         hi = mid;
     }
     const uint32_t b = heavy[lo];
-    const uint32_t q = h - heavy_offset[lo];
-    if ((q & (2 * stride - 1)) != 0 || q + stride >= piece_count[b])
-      continue;
-    g1_xyzz *p = partials + piece_offset[b] + q;
-    store_xyzz(p, xyzz_add(load_xyzz(p), load_xyzz(p + stride)));
+    const uint32_t np = piece_count[b];
+    g1_xyzz *base = partials + piece_offset[b] + (size_t)(c - chunk_offset[lo]) * HEAVY_CHUNK;
+    const uint32_t left = np - (c - chunk_offset[lo]) * HEAVY_CHUNK; // pieces from the chunk start
+    g1_xyzz acc = xyzz_infinity();
+    for (uint32_t k = 0; k < HEAVY_PER_THREAD; k++) {
+      const uint32_t q = threadIdx.x * HEAVY_PER_THREAD + k;
+      if (q < left)
+        acc = xyzz_add(acc, load_xyzz(base + q));
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned half = HEAVY_BLOCK / 2; half > 0; half >>= 1) {
+      if (threadIdx.x < half)
+        sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + half]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      store_xyzz(base, sh[0]);
+    __syncthreads();
+  }
+}
+
+// 4c. one thread per heavy bucket: its chunk sums, in order, into piece 0
+__global__ __launch_bounds__(ACC_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+                                                                const uint32_t *piece_offset, const uint32_t *piece_count) {
+  const uint32_t count = *heavy_count;
+  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < count; h += gridDim.x * blockDim.x) {
+    const uint32_t b = heavy[h];
+    g1_xyzz *base = partials + piece_offset[b];
+    const uint32_t chunks = (piece_count[b] + HEAVY_CHUNK - 1) / HEAVY_CHUNK;
+    g1_xyzz acc = load_xyzz(base);
+    for (uint32_t q = 1; q < chunks; q++)
+      acc = xyzz_add(acc, load_xyzz(base + (size_t)q * HEAVY_CHUNK));
+    store_xyzz(base, acc);
   }
 }
 
@@ -552,22 +591,22 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
 
   // 4. fix-up of multi-piece buckets
   {
-    uint32_t *heavy = nullptr, *heavy_offset = nullptr;
+    uint32_t *heavy = nullptr, *chunk_offset = nullptr;
     BC_TRY(ar.get(&heavy, nb));
-    BC_TRY(ar.get(&heavy_offset, nb));
+    BC_TRY(ar.get(&chunk_offset, nb));
     BC_TRY(fill_u32(meta + 1, 2, 0, s));
     small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta + 1);
     BC_LAUNCH_CHECK();
     if (max_np > SMALL_PIECES) {
-      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta + 1, piece_count, heavy_offset, meta + 2);
+      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta + 1, piece_count, chunk_offset, meta + 2);
       BC_LAUNCH_CHECK();
-      for (unsigned stride = 1; stride < max_np; stride <<= 1) {
-        heavy_tree_kernel<<<1024, ACC_BLOCK, 0, s>>>(partials, heavy, meta + 1, heavy_offset, meta + 2, piece_offset, piece_count, stride);
-        BC_LAUNCH_CHECK();
-      }
+      heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta + 1, chunk_offset, meta + 2, piece_offset, piece_count);
+      BC_LAUNCH_CHECK();
+      heavy_final_kernel<<<16, ACC_BLOCK, 0, s>>>(partials, heavy, meta + 1, piece_offset, piece_count);
+      BC_LAUNCH_CHECK();
     }
     BC_TRY(ar.put(heavy));
-    BC_TRY(ar.put(heavy_offset));
+    BC_TRY(ar.put(chunk_offset));
   }
 
   g1_xyzz *A = nullptr, *tops = nullptr;

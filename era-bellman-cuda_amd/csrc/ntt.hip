@@ -36,7 +36,7 @@ namespace bc {
 namespace {
 
 constexpr unsigned NTT_LOG_TILE = 12; // 4096 elements per tile
-constexpr unsigned NTT_BLOCK = 512;
+constexpr unsigned NTT_BLOCK = 1024;
 constexpr unsigned TWIST_FINE_BITS = 12;
 
 struct twiddle_state {

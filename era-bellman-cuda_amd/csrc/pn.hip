@@ -85,7 +85,7 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   BC_TRY(ar.get(&scells, n));
   pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, keys, cells, cols, L);
   BC_LAUNCH_CHECK();
-  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, 0, bits, ar));
+  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, bits, ar));
   BC_TRY(ar.put(keys));
   BC_TRY(ar.put(cells));
   BC_TRY(ar.get(&flags, n));

@@ -5,7 +5,8 @@
 // (bases, twiddles and power tables stay resident between jobs).  A loop
 // thread owns `slots` streams.  Whenever a slot is free it admits the longest
 // FIFO prefix of waiting jobs whose summed footprint fits the HBM budget left
-// by the running batches and launches it as one batch on that slot's stream:
+// by the running batches (at most an even share of the waiting jobs per free
+// slot) and launches it as one batch on that slot's stream:
 // MSMs and NTTs as batched launches, PN jobs one after another.  Batches on
 // different slots run concurrently, so jobs that arrive while others are on the
 // GPU start as soon as a slot and memory free up.  Completion is observed by
@@ -154,12 +155,15 @@ public:
   }
 
 private:
-  // Longest FIFO prefix that fits the free memory and the per-launch limits.
-  bool take_batch(batch &b) {
+  // Longest FIFO prefix that fits the free memory and the per-launch limits,
+  // and takes no more than its share of the waiting jobs when several slots
+  // are free (so that the free slots' batches run side by side).
+  bool take_batch(batch &b, unsigned free_slots) {
     size_t bytes = 0;
     unsigned n_msm = 0, n_ntt = 0, n_pn = 0;
     uint64_t entries = 0;
-    while (!pending_.empty()) {
+    const size_t share = (pending_.size() + free_slots - 1) / free_slots;
+    while (!pending_.empty() && b.reqs.size() < share) {
       request *r = pending_.front();
       const bool alone = inflight_.empty() && b.reqs.empty();
       if (!alone && used_ + bytes + r->footprint > budget_)
@@ -242,15 +246,17 @@ private:
       if (progress)
         cv_done_.notify_all();
       int slot = -1;
+      unsigned free_slots = 0;
       for (unsigned i = 0; i < slots_; i++)
         if (!busy_[i]) {
-          slot = (int)i;
-          break;
+          if (slot < 0)
+            slot = (int)i;
+          free_slots++;
         }
       if (slot >= 0 && !pending_.empty()) {
         batch b;
         b.slot = (unsigned)slot;
-        if (take_batch(b)) {
+        if (take_batch(b, free_slots)) {
           used_ += b.bytes;
           busy_[b.slot] = true;
           lk.unlock();

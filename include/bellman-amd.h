@@ -115,8 +115,9 @@ bc_error bc_amd_plan_destroy(bc_amd_plan plan);
 /* Server: a request loop on its own thread.  Submitted jobs wait in a FIFO; */
 /* whenever one of `max_in_flight` stream slots is free, the loop admits the */
 /* longest FIFO prefix whose HBM footprint fits next to the batches already  */
-/* running, and launches it as one batch (MSMs and NTTs batched, PN jobs in  */
-/* sequence) on that slot's stream.  Batches on different slots overlap.     */
+/* running (at most an even share of the waiting jobs per free slot), and    */
+/* launches it as one batch (MSMs and NTTs batched, PN jobs in sequence) on  */
+/* that slot's stream.  Batches on different slots overlap.                  */
 
 typedef enum { BC_AMD_JOB_MSM = 0, BC_AMD_JOB_NTT = 1, BC_AMD_JOB_PN = 2 } bc_amd_job_kind;
 
