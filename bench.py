@@ -511,6 +511,10 @@ def served(lab, n_jobs, budget_gib, slots):
         sj.clear()
         st = run_served(srv, sj)
         cap_s = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
+        t0 = min(s.submit_ns for s in st)
+        batches = sorted({(s.start_ns, s.done_ns, s.batch_jobs) for s in st})
+        timeline = [[round((a - t0) / 1e6, 2), round((d - t0) / 1e6, 2), n] for a, d, n in batches]
+        submit_span = (max(s.submit_ns for s in st) - t0) / 1e6
         fp_served = sj.fingerprints()
         mismatches = [j for j in range(n) if fp_served[j] != fp_plain[j]]
         capacity = n / cap_s
@@ -548,6 +552,8 @@ def served(lab, n_jobs, budget_gib, slots):
         "one_at_a_time_jobs_per_s": round(n / plain_s, 1),
         "served_capacity_jobs_per_s": round(capacity, 1),
         "speedup_vs_one_at_a_time": round(capacity / (n / plain_s), 2),
+        "capacity_batches_ms": timeline,
+        "capacity_submit_span_ms": round(submit_span, 2),
         "latency_under_poisson": loads,
         "one_at_a_time_fifo_at_0.7": {"p50_ms": round(pct(lat_plain, 50), 2), "p99_ms": round(pct(lat_plain, 99), 2)},
         "parity": {"served_equal_plain": not mismatches, "mismatched_jobs": mismatches[:8], "msm_closed_form": bool(closed_ok)},
@@ -606,7 +612,7 @@ def main():
     ap.add_argument("--parts", default="tune,ntt,msm,batched,served,cpu",
                     help="comma list of: tune, ntt, msm, batched, served, cpu, copy (a 2^24-element copy kernel, for counter scaling)")
     ap.add_argument("--served-jobs", type=int, default=120)
-    ap.add_argument("--served-budget-gib", type=float, default=64.0)
+    ap.add_argument("--served-budget-gib", type=float, default=160.0)
     ap.add_argument("--served-slots", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -139,14 +139,40 @@ size_t pn_footprint(unsigned cols, unsigned log_rows) {
 
 namespace {
 
-// Median of three timed runs after one warm-up, in milliseconds.
+// A pool that keeps freed memory, so that timed runs do not include the
+// operating system's page allocation (the device's default pool returns memory
+// at every synchronisation).
+struct tuning_pool {
+  hipMemPool_t pool = nullptr;
+  hipStream_t stream = nullptr;
+  hipError_t create(hipStream_t s) {
+    stream = s;
+    int device = 0;
+    BC_TRY(hipGetDevice(&device));
+    hipMemPoolProps props{};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = device;
+    BC_TRY(hipMemPoolCreate(&pool, &props));
+    uint64_t threshold = UINT64_MAX;
+    return hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  }
+  ~tuning_pool() {
+    (void)hipStreamSynchronize(stream);
+    if (pool)
+      (void)hipMemPoolDestroy(pool);
+  }
+};
+
+// Median of five timed runs after one warm-up, in milliseconds.
 template <class F> hipError_t time_median(hipStream_t s, F run, float *ms) {
+  constexpr int RUNS = 5;
   hipEvent_t a, b;
   BC_TRY(hipEventCreate(&a));
   BC_TRY(hipEventCreate(&b));
-  float t[3];
+  float t[RUNS];
   hipError_t e = run();
-  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+  for (int k = 0; k < RUNS && e == hipSuccess; k++) {
     e = hipEventRecord(a, s);
     if (e == hipSuccess)
       e = run();
@@ -160,47 +186,66 @@ template <class F> hipError_t time_median(hipStream_t s, F run, float *ms) {
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   BC_TRY(e);
-  std::sort(t, t + 3);
-  *ms = t[1];
+  std::sort(t, t + RUNS);
+  *ms = t[RUNS / 2];
   return hipSuccess;
 }
 
 hipError_t msm_autotune(const void *bases, const void *scalars, unsigned log_min, unsigned log_max, hipStream_t s) {
   if (log_min > log_max || log_max > 26 || !bases || !scalars)
     return hipErrorInvalidValue;
+  // Sizes below 2^18 are timed as a batch of copies (up to 16): batched and
+  // served callers care about throughput, which a single small MSM's latency
+  // (mostly fixed launch cost) does not show.
+  constexpr unsigned TUNE_COPIES = 16;
+  tuning_pool tp;
+  BC_TRY(tp.create(s));
   g1_jacobian *results = nullptr;
-  BC_TRY(hipMalloc(&results, sizeof(g1_jacobian) * 254));
+  BC_TRY(hipMalloc(&results, sizeof(g1_jacobian) * 254 * TUNE_COPIES));
   hipError_t status = hipSuccess;
   for (unsigned L = log_min; L <= log_max && status == hipSuccess; L++) {
+    const unsigned copies = L >= 18 ? 1u : std::min(TUNE_COPIES, 1u << (18 - L));
     auto timed = [&](unsigned c, unsigned piece, float *ms) {
-      const msm_job job{static_cast<const g1_affine *>(bases), static_cast<const fe *>(scalars), results, L, c, piece};
-      return time_median(
+      std::vector<msm_job> jobs(copies);
+      for (unsigned k = 0; k < copies; k++)
+        jobs[k] = msm_job{static_cast<const g1_affine *>(bases), static_cast<const fe *>(scalars), results + 254 * k, L, c, piece};
+      const hipError_t e = time_median(
           s,
           [&]() {
-            arena ar = arena::pool_mode(nullptr, s);
-            return msm_batch_device(&job, 1, ar);
+            arena ar = arena::pool_mode(tp.pool, s);
+            return msm_batch_device(jobs.data(), copies, ar);
           },
           ms);
+      *ms /= copies;
+      return e;
     };
+    // The built-in choice is the incumbent; a candidate replaces the best so
+    // far only when it is faster by more than the run-to-run noise (2%).
+    constexpr float MARGIN = 0.98f;
     const unsigned c0 = msm_window_bits(L);
     const unsigned c_lo = std::max(4u, c0 > 6 ? c0 - 6 : 4u), c_hi = std::min(22u, c0 + 2);
     unsigned best_c = c0;
     float best = 1e30f;
+    status = timed(c0, MSM_DEFAULT_PIECE, &best);
     for (unsigned c = c_lo; c <= c_hi && status == hipSuccess; c++) {
+      if (c == c0)
+        continue;
       float ms = 0;
       status = timed(c, MSM_DEFAULT_PIECE, &ms);
-      if (status == hipSuccess && ms < best) {
+      if (status == hipSuccess && ms < best * MARGIN) {
         best = ms;
         best_c = c;
       }
     }
     unsigned best_piece = MSM_DEFAULT_PIECE;
-    for (unsigned piece : {8u, 16u, 64u, 128u}) {
+    for (unsigned piece : {8u, 16u, 32u, 64u, 128u}) {
       if (status != hipSuccess)
         break;
+      if (piece == MSM_DEFAULT_PIECE)
+        continue;
       float ms = 0;
       status = timed(best_c, piece, &ms);
-      if (status == hipSuccess && ms < best) {
+      if (status == hipSuccess && ms < best * MARGIN) {
         best = ms;
         best_piece = piece;
       }
@@ -217,6 +262,8 @@ hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipSt
   if (log_min > log_max || log_max > NTT_MAX_LOG || !scratch)
     return hipErrorInvalidValue;
   BC_TRY(ntt_prepare(std::min(log_max, 24u), s, nullptr));
+  tuning_pool tp;
+  BC_TRY(tp.create(s));
   fe *buf = static_cast<fe *>(scratch);
   for (unsigned L = std::max(log_min, 13u); L <= std::min(log_max, 24u); L++) {
     unsigned best_first = 0;
@@ -227,7 +274,7 @@ hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipSt
       BC_TRY(time_median(
           s,
           [&]() {
-            arena ar = arena::pool_mode(nullptr, s);
+            arena ar = arena::pool_mode(tp.pool, s);
             const ntt_job pair[2] = {{buf, buf, L, false, false, 0, 0}, {buf, buf, L, true, true, 0, 0}};
             BC_TRY(ntt_batch_device(&pair[0], 1, ar));
             return ntt_batch_device(&pair[1], 1, ar);

@@ -7,7 +7,7 @@
 
 namespace bc {
 
-constexpr unsigned MSM_DEFAULT_PIECE = 32;
+constexpr unsigned MSM_DEFAULT_PIECE = 64; // points per accumulating thread of the plain call
 constexpr unsigned MSM_MAX_JOBS = 64;
 
 // Window width table of the reference (src/msm.cu:12-37).

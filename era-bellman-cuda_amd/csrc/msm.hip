@@ -27,9 +27,9 @@
 //                  thread accumulates one piece with XYZZ mixed additions, so a
 //                  heavy bucket (e.g. digit 1 of window 0 when many scalars are
 //                  1) is spread over many threads;
-//   4. fix-up:     buckets of up to 8 pieces are summed by one thread; heavier
-//                  buckets are listed and reduced by blocks (chunks of 2048
-//                  pieces: serial sums of 8, then an LDS tree);
+//   4. fix-up:     buckets of up to 32 pieces are summed by one thread;
+//                  heavier buckets are listed and reduced by one-wave blocks
+//                  (chunks of 512 pieces: serial sums of 8, then an LDS tree);
 //   5. per-bit reduction: windows are split recursively into low/high halves
 //                  of their index bits until each holds one bit (sums split
 //                  across threads, then folded pairwise);
@@ -78,13 +78,14 @@ constexpr uint32_t SIGN = 0x80000000u;
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
 constexpr unsigned SPLIT_TERMS = 16;
-constexpr unsigned SMALL_PIECES = 8;
+constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many pieces are summed by one thread
 
 struct job_desc {
   const g1_affine *bases;
   const fe *scalars;
   uint64_t entry_begin; // first (key, value) pair of the job
   uint32_t n;           // scalars
+  uint32_t log_n;
   uint32_t c;           // window bits
   uint32_t W;           // windows
   uint32_t scalar_begin;
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table 
   const uint32_t top = 1u << (jd.c - 1);
   if (k == top)
     return; // zero digits
-  const uint32_t local = e - (uint32_t)jd.entry_begin, w = local / jd.n;
+  const uint32_t local = e - (uint32_t)jd.entry_begin, w = local >> jd.log_n;
   const uint32_t seg_begin = (uint32_t)jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
   const uint32_t b = jd.bucket_begin + w * top + k;
   if (e == seg_begin || keys[e - 1] != k)
@@ -200,15 +201,34 @@ __global__ void piece_count_kernel(const job_table jobs, const uint32_t *start, 
   pieces[b] = (cnt + piece - 1) / piece;
 }
 
-__global__ void piece_map_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket, unsigned nb, uint32_t *total) {
+// piece -> bucket map: one thread writes a bucket's first PIECE_MAP_DIRECT
+// entries; buckets with more pieces (a few giant buckets, e.g. the last window
+// when it keeps only 2 bits) are listed and filled by piece_map_big_kernel.
+constexpr unsigned PIECE_MAP_DIRECT = 64;
+
+__global__ void piece_map_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket, unsigned nb, uint32_t *total,
+                                 uint32_t *big, uint32_t *big_count) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb)
     return;
   const uint32_t np = piece_count[b], off = piece_offset[b];
-  for (uint32_t q = 0; q < np; q++)
+  const uint32_t direct = np < PIECE_MAP_DIRECT ? np : PIECE_MAP_DIRECT;
+  for (uint32_t q = 0; q < direct; q++)
     piece_bucket[off + q] = b;
+  if (np > PIECE_MAP_DIRECT)
+    big[atomicAdd(big_count, 1u)] = b;
   if (b == nb - 1)
     *total = off + np;
+}
+
+__global__ __launch_bounds__(256) void piece_map_big_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket,
+                                                            const uint32_t *big, const uint32_t *big_count) {
+  const uint32_t count = *big_count;
+  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
+    const uint32_t b = big[i], np = piece_count[b], off = piece_offset[b];
+    for (uint32_t q = PIECE_MAP_DIRECT + threadIdx.x; q < np; q += blockDim.x)
+      piece_bucket[off + q] = b;
+  }
 }
 
 // 3. one thread accumulates one piece (<= piece points of one bucket)
@@ -246,7 +266,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
 }
 
 // 4a. buckets of 2..SMALL_PIECES pieces are summed serially into piece 0;
-// larger ("heavy") buckets are listed for the tree below.
+// larger ("heavy") buckets are listed for the block reduction below.
 __global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count, unsigned nb,
                                                                 uint32_t *heavy, uint32_t *heavy_count) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -270,7 +290,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partial
 // chunk: each thread sums HEAVY_PER_THREAD consecutive pieces, then a tree in
 // LDS; the chunk's sum overwrites the chunk's first piece.  A second kernel
 // adds a bucket's chunk sums in order into its first piece.
-constexpr unsigned HEAVY_BLOCK = 256;
+constexpr unsigned HEAVY_BLOCK = 64;
 constexpr unsigned HEAVY_PER_THREAD = 8;
 constexpr unsigned HEAVY_CHUNK = HEAVY_BLOCK * HEAVY_PER_THREAD;
 
@@ -338,18 +358,29 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
   }
 }
 
-// 4c. one thread per heavy bucket: its chunk sums, in order, into piece 0
-__global__ __launch_bounds__(ACC_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
-                                                                const uint32_t *piece_offset, const uint32_t *piece_count) {
+// 4c. one block per heavy bucket (grid-stride): its chunk sums, strided over
+// the threads, then an LDS tree, into piece 0
+__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+                                                                  const uint32_t *piece_offset, const uint32_t *piece_count) {
+  __shared__ g1_xyzz sh[HEAVY_BLOCK];
   const uint32_t count = *heavy_count;
-  for (uint32_t h = blockIdx.x * blockDim.x + threadIdx.x; h < count; h += gridDim.x * blockDim.x) {
+  for (uint32_t h = blockIdx.x; h < count; h += gridDim.x) {
     const uint32_t b = heavy[h];
     g1_xyzz *base = partials + piece_offset[b];
     const uint32_t chunks = (piece_count[b] + HEAVY_CHUNK - 1) / HEAVY_CHUNK;
-    g1_xyzz acc = load_xyzz(base);
-    for (uint32_t q = 1; q < chunks; q++)
+    g1_xyzz acc = xyzz_infinity();
+    for (uint32_t q = threadIdx.x; q < chunks; q += HEAVY_BLOCK)
       acc = xyzz_add(acc, load_xyzz(base + (size_t)q * HEAVY_CHUNK));
-    store_xyzz(base, acc);
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (unsigned half = HEAVY_BLOCK / 2; half > 0; half >>= 1) {
+      if (threadIdx.x < half)
+        sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + half]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0)
+      store_xyzz(base, sh[0]);
+    __syncthreads();
   }
 }
 
@@ -363,7 +394,7 @@ __global__ void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1
     return;
   const job_desc &jd = jobs.job[find_job(jobs, t, by_bucket())];
   const uint32_t B = 1u << (jd.c - 1);
-  const uint32_t local = t - jd.bucket_begin, w = local / B, k = local % B;
+  const uint32_t local = t - jd.bucket_begin, w = local >> (jd.c - 1), k = local & (B - 1);
   const uint32_t top_signed = jd.c - (jd.W * jd.c - 254) - 1;
   const uint32_t top_digit = w == jd.W - 1 ? (1u << top_signed) : B;
   auto bucket_value = [&](uint32_t d) {
@@ -525,6 +556,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     d.bases = jobs[j].bases;
     d.scalars = jobs[j].scalars;
     d.n = 1u << jobs[j].log_n;
+    d.log_n = jobs[j].log_n;
     d.c = c;
     d.W = (254 - 1) / c + 1;
     d.scalar_begin = scalars_total;
@@ -573,19 +605,24 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     BC_TRY(ar.put(sorted_keys));
   }
 
-  // 3. pieces and accumulation; meta = {piece total, heavy count, heavy piece total}
-  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *meta = nullptr;
+  // 3. pieces and accumulation; meta = {piece total, heavy count, heavy chunk total, big count}
+  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *meta = nullptr, *big = nullptr;
   g1_xyzz *partials = nullptr;
   BC_TRY(ar.get(&piece_count, nb));
   BC_TRY(ar.get(&piece_offset, nb));
   BC_TRY(ar.get(&piece_bucket, max_pieces));
-  BC_TRY(ar.get(&meta, 3));
+  BC_TRY(ar.get(&meta, 4));
+  BC_TRY(ar.get(&big, nb));
   BC_TRY(ar.get(&partials, max_pieces));
+  BC_TRY(fill_u32(meta, 4, 0, s));
   piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_count, nb);
   BC_LAUNCH_CHECK();
   BC_TRY(scan_u32_exclusive(piece_count, piece_offset, nb, ar));
-  piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, meta);
+  piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, meta, big, meta + 3);
   BC_LAUNCH_CHECK();
+  piece_map_big_kernel<<<256, 256, 0, s>>>(piece_count, piece_offset, piece_bucket, big, meta + 3);
+  BC_LAUNCH_CHECK();
+  BC_TRY(ar.put(big));
   accumulate_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, offsets, counts, piece_offset, piece_bucket, meta, partials);
   BC_LAUNCH_CHECK();
 
@@ -594,7 +631,6 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     uint32_t *heavy = nullptr, *chunk_offset = nullptr;
     BC_TRY(ar.get(&heavy, nb));
     BC_TRY(ar.get(&chunk_offset, nb));
-    BC_TRY(fill_u32(meta + 1, 2, 0, s));
     small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta + 1);
     BC_LAUNCH_CHECK();
     if (max_np > SMALL_PIECES) {
@@ -602,7 +638,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
       BC_LAUNCH_CHECK();
       heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta + 1, chunk_offset, meta + 2, piece_offset, piece_count);
       BC_LAUNCH_CHECK();
-      heavy_final_kernel<<<16, ACC_BLOCK, 0, s>>>(partials, heavy, meta + 1, piece_offset, piece_count);
+      heavy_final_kernel<<<256, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta + 1, piece_offset, piece_count);
       BC_LAUNCH_CHECK();
     }
     BC_TRY(ar.put(heavy));

@@ -77,20 +77,24 @@ class server {
 public:
   server(int device, size_t budget, unsigned slots) : device_(device), budget_(budget), slots_(slots) {}
 
+  // Each slot has its own stream and memory pool: a pool shared by the slots
+  // would hand memory freed on one slot's stream to another only after making
+  // that stream wait for the first, which serialises the batches.
   hipError_t start() {
     hipMemPoolProps props{};
     props.allocType = hipMemAllocationTypePinned;
     props.location.type = hipMemLocationTypeDevice;
     props.location.id = device_;
-    BC_TRY(hipMemPoolCreate(&pool_, &props));
-    uint64_t threshold = UINT64_MAX; // keep freed memory reserved for the next batches
-    BC_TRY(hipMemPoolSetAttribute(pool_, hipMemPoolAttrReleaseThreshold, &threshold));
     streams_.resize(slots_, nullptr);
     events_.resize(slots_, nullptr);
+    pools_.resize(slots_, nullptr);
     busy_.assign(slots_, false);
     for (unsigned i = 0; i < slots_; i++) {
       BC_TRY(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
       BC_TRY(hipEventCreateWithFlags(&events_[i], hipEventDisableTiming));
+      BC_TRY(hipMemPoolCreate(&pools_[i], &props));
+      uint64_t threshold = UINT64_MAX; // keep freed memory reserved for the slot's next batches
+      BC_TRY(hipMemPoolSetAttribute(pools_[i], hipMemPoolAttrReleaseThreshold, &threshold));
     }
     thread_ = std::thread([this] { loop(); });
     return hipSuccess;
@@ -111,8 +115,9 @@ public:
     for (hipEvent_t e : events_)
       if (e)
         (void)hipEventDestroy(e);
-    if (pool_)
-      (void)hipMemPoolDestroy(pool_);
+    for (hipMemPool_t p : pools_)
+      if (p)
+        (void)hipMemPoolDestroy(p);
     for (auto &kv : all_)
       delete kv.second;
   }
@@ -207,7 +212,7 @@ private:
       else
         pn.push_back(&r->pn);
     }
-    arena ar = arena::pool_mode(pool_, s);
+    arena ar = arena::pool_mode(pools_[b.slot], s);
     BC_TRY(msm_run(msm, ar));
     BC_TRY(ntt_run(ntt, ar));
     for (const bc_amd_pn_job *p : pn)
@@ -286,7 +291,7 @@ private:
   size_t budget_;
   unsigned slots_;
   size_t used_ = 0;
-  hipMemPool_t pool_ = nullptr;
+  std::vector<hipMemPool_t> pools_;
   std::vector<hipStream_t> streams_;
   std::vector<hipEvent_t> events_;
   std::vector<bool> busy_;
