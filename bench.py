@@ -512,8 +512,8 @@ def served(lab, n_jobs, budget_gib, slots):
         st = run_served(srv, sj)
         cap_s = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
         t0 = min(s.submit_ns for s in st)
-        batches = sorted({(s.start_ns, s.done_ns, s.batch_jobs) for s in st})
-        timeline = [[round((a - t0) / 1e6, 2), round((d - t0) / 1e6, 2), n] for a, d, n in batches]
+        batches = sorted({(s.start_ns, s.enqueued_ns, s.done_ns, s.batch_jobs, s.slot) for s in st})
+        timeline = [[round((a - t0) / 1e6, 2), round((q - t0) / 1e6, 2), round((d - t0) / 1e6, 2), n, k] for a, q, d, n, k in batches]
         submit_span = (max(s.submit_ns for s in st) - t0) / 1e6
         fp_served = sj.fingerprints()
         mismatches = [j for j in range(n) if fp_served[j] != fp_plain[j]]

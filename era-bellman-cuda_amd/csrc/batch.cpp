@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "batch.h"
@@ -24,12 +25,29 @@ namespace {
 
 constexpr unsigned TUNE_MAX_LOG = 28;
 
+// Built-in window width and piece size of batched MSM jobs, per log size: the
+// fastest per-job time of batches of equal jobs measured on MI355X (copies of
+// one job up to 2^22 points per batch; single jobs above).  Narrower windows
+// than the plain call's pay off in batches: fewer buckets to reduce per job.
+msm_tuning msm_builtin_tuning(unsigned l) {
+  static const msm_tuning measured[] = {
+      {5, 32},  /* 12 */ {6, 16},  /* 13 */ {6, 32},  /* 14 */ {9, 16},  /* 15 */ {11, 16}, /* 16 */
+      {11, 16}, /* 17 */ {11, 16}, /* 18 */ {12, 16}, /* 19 */ {13, 16}, /* 20 */ {14, 16}, /* 21 */
+      {15, 16}, /* 22 */ {15, 32}, /* 23 */ {17, 32}, /* 24 */
+  };
+  if (l < 12)
+    return {std::max(3u, l > 7 ? l - 7 : 3u), 16};
+  if (l <= 24)
+    return measured[l - 12];
+  return {std::min(msm_window_bits(l), 22u), 32};
+}
+
 struct tuning_table {
   msm_tuning msm[TUNE_MAX_LOG + 1];
   unsigned ntt_first[TUNE_MAX_LOG + 1];
   tuning_table() {
     for (unsigned l = 0; l <= TUNE_MAX_LOG; l++) {
-      msm[l] = {std::min(msm_window_bits(l), 22u), MSM_DEFAULT_PIECE};
+      msm[l] = msm_builtin_tuning(l);
       ntt_first[l] = 0;
     }
   }
@@ -87,8 +105,12 @@ ntt_job resolve_ntt(const bc_amd_ntt_job &in) {
                  in.inverse, in.log_extension_degree, in.coset_index};
 }
 
-// Runs MSM jobs in launches of at most MSM_MAX_JOBS jobs and 2^31 - 1 digit entries.
-hipError_t msm_run(const std::vector<msm_job> &jobs, arena &ar) {
+namespace {
+
+// Launch boundaries of msm_run: at most MSM_MAX_JOBS jobs and 2^31 - 1 digit
+// entries per launch.
+std::vector<std::pair<size_t, size_t>> msm_chunks(const std::vector<msm_job> &jobs) {
+  std::vector<std::pair<size_t, size_t>> out;
   size_t i = 0;
   while (i < jobs.size()) {
     size_t j = i;
@@ -100,9 +122,17 @@ hipError_t msm_run(const std::vector<msm_job> &jobs, arena &ar) {
       entries += e;
       j++;
     }
-    BC_TRY(msm_batch_device(jobs.data() + i, (unsigned)(j - i), ar));
+    out.push_back({i, j});
     i = j;
   }
+  return out;
+}
+
+} // namespace
+
+hipError_t msm_run(const std::vector<msm_job> &jobs, arena &ar) {
+  for (const auto &c : msm_chunks(jobs))
+    BC_TRY(msm_batch_device(jobs.data() + c.first, (unsigned)(c.second - c.first), ar));
   return hipSuccess;
 }
 
@@ -110,6 +140,38 @@ hipError_t ntt_run(const std::vector<ntt_job> &jobs, arena &ar) {
   for (size_t i = 0; i < jobs.size(); i += NTT_MAX_JOBS)
     BC_TRY(ntt_batch_device(jobs.data() + i, (unsigned)std::min<size_t>(NTT_MAX_JOBS, jobs.size() - i), ar));
   return hipSuccess;
+}
+
+size_t msm_run_footprint(const std::vector<msm_job> &jobs) {
+  size_t most = 0;
+  for (const auto &c : msm_chunks(jobs)) {
+    size_t sum = 0;
+    for (size_t k = c.first; k < c.second; k++)
+      sum += msm_footprint(jobs[k]);
+    most = std::max(most, sum);
+  }
+  return most;
+}
+
+size_t ntt_run_footprint(const std::vector<ntt_job> &jobs) {
+  size_t most = 0;
+  for (size_t i = 0; i < jobs.size(); i += NTT_MAX_JOBS) {
+    size_t sum = 0;
+    for (size_t k = i; k < std::min<size_t>(jobs.size(), i + NTT_MAX_JOBS); k++)
+      sum += ntt_footprint(jobs[k]);
+    most = std::max(most, sum);
+  }
+  return most;
+}
+
+hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const std::function<hipError_t(arena &)> &record) {
+  char *block = nullptr;
+  BC_TRY(pool_alloc(&block, std::max<size_t>(bytes, 256), pool, s));
+  arena ar = arena::bump_mode(block, bytes, s);
+  const hipError_t e = record(ar);
+  const hipError_t f = pool_free(block, s); // ordered after whatever `record` enqueued
+  BC_TRY(e);
+  return f;
 }
 
 // Temporaries of msm_batch_device for one job: 6 x 4 B per digit entry (keys,
@@ -135,32 +197,50 @@ size_t pn_footprint(unsigned cols, unsigned log_rows) {
   return (size_t)(53 * n + (1u << 20));
 }
 
+hipError_t measure_scratch(hipStream_t s, const std::function<hipError_t(arena &)> &record, size_t *bytes) {
+  arena sizing = arena::bump_mode(reinterpret_cast<void *>(uintptr_t(1) << 44), size_t(1) << 43, s);
+  BC_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const hipError_t e = record(sizing);
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (g)
+    (void)hipGraphDestroy(g);
+  BC_TRY(e);
+  BC_TRY(end);
+  *bytes = sizing.peak();
+  return hipSuccess;
+}
+
 // ---------------------------------------------------------------- autotune
 
 namespace {
 
-// A pool that keeps freed memory, so that timed runs do not include the
-// operating system's page allocation (the device's default pool returns memory
-// at every synchronisation).
-struct tuning_pool {
-  hipMemPool_t pool = nullptr;
+// Scratch of the timed runs: one workspace, grown (outside the timed region)
+// to the largest exact need so far.  Timing from a memory pool would include
+// the pool's growth whenever a configuration needs blocks of a new size.
+struct tuning_workspace {
   hipStream_t stream = nullptr;
-  hipError_t create(hipStream_t s) {
-    stream = s;
-    int device = 0;
-    BC_TRY(hipGetDevice(&device));
-    hipMemPoolProps props{};
-    props.allocType = hipMemAllocationTypePinned;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = device;
-    BC_TRY(hipMemPoolCreate(&pool, &props));
-    uint64_t threshold = UINT64_MAX;
-    return hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  void *base = nullptr;
+  size_t bytes = 0;
+  explicit tuning_workspace(hipStream_t s) : stream(s) {}
+  hipError_t fit(const std::function<hipError_t(arena &)> &record) {
+    size_t need = 0;
+    BC_TRY(measure_scratch(stream, record, &need));
+    if (need <= bytes)
+      return hipSuccess;
+    BC_TRY(hipStreamSynchronize(stream));
+    if (base)
+      BC_TRY(hipFree(base));
+    base = nullptr;
+    bytes = 0;
+    BC_TRY(hipMalloc(&base, need));
+    bytes = need;
+    return hipSuccess;
   }
-  ~tuning_pool() {
+  ~tuning_workspace() {
     (void)hipStreamSynchronize(stream);
-    if (pool)
-      (void)hipMemPoolDestroy(pool);
+    if (base)
+      (void)hipFree(base);
   }
 };
 
@@ -198,8 +278,7 @@ hipError_t msm_autotune(const void *bases, const void *scalars, unsigned log_min
   // served callers care about throughput, which a single small MSM's latency
   // (mostly fixed launch cost) does not show.
   constexpr unsigned TUNE_COPIES = 16;
-  tuning_pool tp;
-  BC_TRY(tp.create(s));
+  tuning_workspace ws(s);
   g1_jacobian *results = nullptr;
   BC_TRY(hipMalloc(&results, sizeof(g1_jacobian) * 254 * TUNE_COPIES));
   hipError_t status = hipSuccess;
@@ -209,11 +288,13 @@ hipError_t msm_autotune(const void *bases, const void *scalars, unsigned log_min
       std::vector<msm_job> jobs(copies);
       for (unsigned k = 0; k < copies; k++)
         jobs[k] = msm_job{static_cast<const g1_affine *>(bases), static_cast<const fe *>(scalars), results + 254 * k, L, c, piece};
+      auto record = [&](arena &ar) { return msm_batch_device(jobs.data(), copies, ar); };
+      BC_TRY(ws.fit(record));
       const hipError_t e = time_median(
           s,
           [&]() {
-            arena ar = arena::pool_mode(tp.pool, s);
-            return msm_batch_device(jobs.data(), copies, ar);
+            arena ar = arena::bump_mode(ws.base, ws.bytes, s);
+            return record(ar);
           },
           ms);
       *ms /= copies;
@@ -222,26 +303,27 @@ hipError_t msm_autotune(const void *bases, const void *scalars, unsigned log_min
     // The built-in choice is the incumbent; a candidate replaces the best so
     // far only when it is faster by more than the run-to-run noise (2%).
     constexpr float MARGIN = 0.98f;
-    const unsigned c0 = msm_window_bits(L);
-    const unsigned c_lo = std::max(4u, c0 > 6 ? c0 - 6 : 4u), c_hi = std::min(22u, c0 + 2);
+    const msm_tuning incumbent = msm_tuned(L);
+    const unsigned c0 = incumbent.c, piece0 = incumbent.piece;
+    const unsigned c_lo = c0 > 6 ? c0 - 3 : 3u, c_hi = std::min(22u, c0 + 3);
     unsigned best_c = c0;
     float best = 1e30f;
-    status = timed(c0, MSM_DEFAULT_PIECE, &best);
+    status = timed(c0, piece0, &best);
     for (unsigned c = c_lo; c <= c_hi && status == hipSuccess; c++) {
       if (c == c0)
         continue;
       float ms = 0;
-      status = timed(c, MSM_DEFAULT_PIECE, &ms);
+      status = timed(c, piece0, &ms);
       if (status == hipSuccess && ms < best * MARGIN) {
         best = ms;
         best_c = c;
       }
     }
-    unsigned best_piece = MSM_DEFAULT_PIECE;
+    unsigned best_piece = piece0;
     for (unsigned piece : {8u, 16u, 32u, 64u, 128u}) {
       if (status != hipSuccess)
         break;
-      if (piece == MSM_DEFAULT_PIECE)
+      if (piece == piece0)
         continue;
       float ms = 0;
       status = timed(best_c, piece, &ms);
@@ -262,8 +344,7 @@ hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipSt
   if (log_min > log_max || log_max > NTT_MAX_LOG || !scratch)
     return hipErrorInvalidValue;
   BC_TRY(ntt_prepare(std::min(log_max, 24u), s, nullptr));
-  tuning_pool tp;
-  BC_TRY(tp.create(s));
+  tuning_workspace ws(s);
   fe *buf = static_cast<fe *>(scratch);
   for (unsigned L = std::max(log_min, 13u); L <= std::min(log_max, 24u); L++) {
     unsigned best_first = 0;
@@ -271,13 +352,17 @@ hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipSt
     for (unsigned first = L - 12; first <= 12; first++) {
       ntt_set_first_pass(L, first);
       float ms = 0;
+      auto record = [&](arena &ar) {
+        const ntt_job pair[2] = {{buf, buf, L, false, false, 0, 0}, {buf, buf, L, true, true, 0, 0}};
+        BC_TRY(ntt_batch_device(&pair[0], 1, ar));
+        return ntt_batch_device(&pair[1], 1, ar);
+      };
+      BC_TRY(ws.fit(record));
       BC_TRY(time_median(
           s,
           [&]() {
-            arena ar = arena::pool_mode(tp.pool, s);
-            const ntt_job pair[2] = {{buf, buf, L, false, false, 0, 0}, {buf, buf, L, true, true, 0, 0}};
-            BC_TRY(ntt_batch_device(&pair[0], 1, ar));
-            return ntt_batch_device(&pair[1], 1, ar);
+            arena ar = arena::bump_mode(ws.base, ws.bytes, s);
+            return record(ar);
           },
           &ms));
       if (ms < best) {
@@ -335,14 +420,8 @@ hipError_t plan_create(plan_state *p) {
   hipError_t e = max_log ? ntt_prepare(max_log, s, nullptr) : hipSuccess;
   // First capture only sizes the workspace: its kernels see placeholder
   // addresses and the graph is destroyed without being launched.
-  if (e == hipSuccess) {
-    arena sizing = arena::bump_mode(reinterpret_cast<void *>(uintptr_t(1) << 44), size_t(1) << 43, s);
-    hipGraph_t g = nullptr;
-    e = capture(*p, sizing, s, &g);
-    if (g)
-      (void)hipGraphDestroy(g);
-    p->bytes = sizing.requested();
-  }
+  if (e == hipSuccess)
+    e = measure_scratch(s, [&](arena &ar) { return record(*p, ar); }, &p->bytes);
   if (e == hipSuccess)
     e = hipMalloc(&p->workspace, std::max<size_t>(p->bytes, 256));
   if (e == hipSuccess) {
@@ -385,8 +464,7 @@ bc_error bc_amd_msm_batch_async(const bc_amd_msm_job *jobs, unsigned count, bc_m
   std::vector<msm_job> v(count);
   for (unsigned i = 0; i < count; i++)
     BC_TRY_BC(resolve_msm(jobs[i], &v[i]));
-  arena ar = arena::pool_mode(to_pool(pool), to_stream(stream));
-  return as_bc(msm_run(v, ar));
+  return as_bc(run_in_block(to_pool(pool), to_stream(stream), msm_run_footprint(v), [&](arena &ar) { return msm_run(v, ar); }));
 }
 
 bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_mem_pool pool, bc_stream stream) {
@@ -395,8 +473,7 @@ bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_m
   std::vector<ntt_job> v(count);
   for (unsigned i = 0; i < count; i++)
     v[i] = resolve_ntt(jobs[i]);
-  arena ar = arena::pool_mode(to_pool(pool), to_stream(stream));
-  return as_bc(ntt_run(v, ar));
+  return as_bc(run_in_block(to_pool(pool), to_stream(stream), ntt_run_footprint(v), [&](arena &ar) { return ntt_run(v, ar); }));
 }
 
 bc_error bc_amd_msm_get_tuning(unsigned log_scalars_count, bc_amd_msm_tuning *tuning) {

@@ -6,6 +6,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/bellman-cuda.h"
 
@@ -49,10 +50,12 @@ template <class T> hipError_t pool_alloc(T **ptr, size_t count, hipMemPool_t poo
 inline hipError_t pool_free(void *ptr, hipStream_t stream) { return ptr ? hipFreeAsync(ptr, stream) : hipSuccess; }
 
 // Scratch memory of one operation.  Pool mode takes stream-ordered
-// allocations from the caller's pool (plain C-ABI calls) and records the total
-// requested; bump mode carves a preallocated workspace (plans, hipGraph
-// capture: no allocation inside the captured region) and fails with
-// hipErrorOutOfMemory instead of overrunning it.
+// allocations from the caller's pool.  Bump mode carves a preallocated
+// workspace as a stack (plans, hipGraph capture, the server, single-block plain
+// calls): a freed block is reclaimed once every block above it is freed too,
+// which is safe because all work on the workspace is ordered on one stream.
+// It fails with hipErrorOutOfMemory instead of overrunning the workspace, and
+// peak() is its high-water mark.
 class arena {
 public:
   static arena pool_mode(hipMemPool_t pool, hipStream_t stream) { return arena(pool, stream, nullptr, 0, false); }
@@ -60,30 +63,49 @@ public:
 
   template <class T> hipError_t get(T **ptr, size_t count) {
     const size_t bytes = ((count * sizeof(T) > 0 ? count * sizeof(T) : 16) + 255) & ~size_t(255);
-    requested_ += bytes;
-    if (bump_) {
-      if (used_ + bytes > size_)
-        return hipErrorOutOfMemory;
-      *ptr = reinterpret_cast<T *>(base_ + used_);
-      used_ += bytes;
-      return hipSuccess;
-    }
-    return pool_alloc(ptr, bytes / sizeof(T) + 1, pool_, stream_);
+    if (!bump_)
+      return pool_alloc(ptr, bytes / sizeof(T) + 1, pool_, stream_);
+    if (used_ + bytes > size_)
+      return hipErrorOutOfMemory;
+    *ptr = reinterpret_cast<T *>(base_ + used_);
+    blocks_.push_back({used_, false});
+    used_ += bytes;
+    peak_ = used_ > peak_ ? used_ : peak_;
+    return hipSuccess;
   }
 
-  hipError_t put(void *ptr) { return bump_ ? hipSuccess : pool_free(ptr, stream_); }
+  hipError_t put(void *ptr) {
+    if (!bump_)
+      return pool_free(ptr, stream_);
+    const size_t off = static_cast<size_t>(static_cast<char *>(ptr) - base_);
+    for (size_t i = blocks_.size(); i-- > 0;)
+      if (blocks_[i].offset == off && !blocks_[i].freed) {
+        blocks_[i].freed = true;
+        break;
+      }
+    while (!blocks_.empty() && blocks_.back().freed) {
+      used_ = blocks_.back().offset;
+      blocks_.pop_back();
+    }
+    return hipSuccess;
+  }
 
   hipStream_t stream() const { return stream_; }
-  size_t requested() const { return requested_; }
+  size_t peak() const { return peak_; }
 
 private:
+  struct block {
+    size_t offset;
+    bool freed;
+  };
   arena(hipMemPool_t pool, hipStream_t stream, char *base, size_t size, bool bump)
-      : pool_(pool), stream_(stream), base_(base), size_(size), used_(0), requested_(0), bump_(bump) {}
+      : pool_(pool), stream_(stream), base_(base), size_(size), used_(0), peak_(0), bump_(bump) {}
   hipMemPool_t pool_;
   hipStream_t stream_;
   char *base_;
-  size_t size_, used_, requested_;
+  size_t size_, used_, peak_;
   bool bump_;
+  std::vector<block> blocks_;
 };
 
 enum class mem_kind { device, host_pinned, host_pageable };

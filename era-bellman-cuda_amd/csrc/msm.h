@@ -7,7 +7,10 @@
 
 namespace bc {
 
-constexpr unsigned MSM_DEFAULT_PIECE = 64; // points per accumulating thread of the plain call
+// Points per accumulating thread of the plain call at the reference window
+// widths: 16 up to 2^23 points, 64 above (measured on MI355X; shorter chains
+// pay off while a launch holds too few threads to hide latency).
+constexpr unsigned msm_plain_piece(unsigned log_n) { return log_n <= 23 ? 16u : 64u; }
 constexpr unsigned MSM_MAX_JOBS = 64;
 
 // Window width table of the reference (src/msm.cu:12-37).

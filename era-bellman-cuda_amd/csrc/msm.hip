@@ -37,6 +37,7 @@
 #include <algorithm>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 #include "curve.cuh"
 #include "msm.h"
@@ -765,9 +766,8 @@ hipError_t msm_execute(const msm_configuration &cfg) {
     BC_TRY(pool_alloc(&tmp_results, 254, pool, s));
     d_results = tmp_results;
   }
-  const msm_job job{static_cast<const g1_affine *>(cfg.bases), d_scalars, d_results, L, msm_window_bits(L), MSM_DEFAULT_PIECE};
-  arena ar = arena::pool_mode(pool, s);
-  BC_TRY(msm_batch_device(&job, 1, ar));
+  const msm_job job{static_cast<const g1_affine *>(cfg.bases), d_scalars, d_results, L, msm_window_bits(L), msm_plain_piece(L)};
+  BC_TRY(run_in_block(pool, s, msm_footprint(job), [&](arena &ar) { return msm_batch_device(&job, 1, ar); }));
   if (results_host) {
     BC_TRY(hipMemcpyAsync(cfg.results, tmp_results, sizeof(g1_jacobian) * 254, hipMemcpyDeviceToHost, s));
     if (cfg.d2h_copy_finished.handle)

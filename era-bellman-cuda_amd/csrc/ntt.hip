@@ -26,6 +26,7 @@
 #include <mutex>
 #include <vector>
 
+#include "batch.h"
 #include "common.h"
 #include "ntt.h"
 #include "tables.h"
@@ -498,8 +499,7 @@ hipError_t ntt_execute(const ntt_configuration &c) {
     }
   }
   const ntt_job job{d_in, d_out, L, c.bit_reversed_inputs, c.inverse, c.log_extension_degree, c.coset_index};
-  arena ar = arena::pool_mode(pool, s);
-  BC_TRY(ntt_batch_device(&job, 1, ar));
+  BC_TRY(run_in_block(pool, s, ntt_footprint(job), [&](arena &ar) { return ntt_batch_device(&job, 1, ar); }));
   if (out_host) {
     BC_TRY(hipMemcpyAsync(c.outputs, d_out, bytes, hipMemcpyDeviceToHost, s));
     if (c.d2h_copy_finished.handle)

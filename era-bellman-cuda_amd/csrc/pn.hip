@@ -7,6 +7,7 @@
 // receives the identity value omega_rows^row * k_col of the NEXT cell of its
 // group (cyclically), except the first group (smallest key), which keeps the
 // identity.  The output is column-major, canonical Montgomery Fr.
+#include "batch.h"
 #include "common.h"
 #include "pn.h"
 #include "scan.h"
@@ -114,9 +115,12 @@ extern "C" {
 bc_error pn_set_up() { return ff_current() ? bc_success : bc_error_invalid_value; }
 
 bc_error pn_generate_permutation_polynomials(generate_permutation_polynomials_configuration configuration) {
-  arena ar = arena::pool_mode(to_pool(configuration.mem_pool), to_stream(configuration.stream));
-  return as_bc(pn_generate_device(configuration.indexes, static_cast<const fe *>(configuration.scalars), static_cast<fe *>(configuration.target),
-                                  configuration.columns_count, configuration.log_rows_count, ar));
+  const generate_permutation_polynomials_configuration &c = configuration;
+  if ((c.columns_count != 3 && c.columns_count != 4) || c.log_rows_count > 28)
+    return bc_error_invalid_value;
+  return as_bc(run_in_block(to_pool(c.mem_pool), to_stream(c.stream), pn_footprint(c.columns_count, c.log_rows_count), [&](arena &ar) {
+    return pn_generate_device(c.indexes, static_cast<const fe *>(c.scalars), static_cast<fe *>(c.target), c.columns_count, c.log_rows_count, ar);
+  }));
 }
 
 bc_error pn_tear_down() { return bc_success; }

@@ -11,9 +11,17 @@
 // different slots run concurrently, so jobs that arrive while others are on the
 // GPU start as soon as a slot and memory free up.  Completion is observed by
 // polling each batch's event.
+//
+// Memory: the server allocates its whole budget once, as one workspace, and
+// gives each batch a region of it sized by a sizing capture of the batch (the
+// launch sequence recorded against placeholder addresses, never run).  No
+// device allocation happens while batches run: growing a memory pool can wait
+// for work queued on other streams, which would serialise the slots.  A batch
+// larger than the workspace runs alone with memory from a pool.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -21,6 +29,7 @@
 
 #include "batch.h"
 #include "common.h"
+#include "ntt.h"
 #include "pn.h"
 
 namespace bc {
@@ -45,7 +54,54 @@ struct request {
 struct batch {
   unsigned slot = 0;
   std::vector<request *> reqs;
-  size_t bytes = 0;
+  size_t bytes = 0;         // summed footprints (admission)
+  size_t region = 0;        // offset of the workspace region
+  size_t region_bytes = 0;  // exact size; 0 when the batch ran from the pool
+};
+
+// First-fit allocator of workspace regions (byte offsets, 256-byte granules).
+class region_heap {
+public:
+  void reset(size_t size) {
+    free_.clear();
+    if (size)
+      free_[0] = size;
+  }
+  bool take(size_t bytes, size_t *offset) {
+    bytes = round(bytes);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < bytes)
+        continue;
+      *offset = it->first;
+      const size_t at = it->first + bytes, rest = it->second - bytes;
+      free_.erase(it);
+      if (rest)
+        free_[at] = rest;
+      return true;
+    }
+    return false;
+  }
+  void give(size_t offset, size_t bytes) {
+    bytes = round(bytes);
+    auto next = free_.lower_bound(offset);
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == offset) {
+        offset = prev->first;
+        bytes += prev->second;
+        free_.erase(prev);
+      }
+    }
+    if (next != free_.end() && offset + bytes == next->first) {
+      bytes += next->second;
+      free_.erase(next);
+    }
+    free_[offset] = bytes;
+  }
+
+private:
+  static size_t round(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~size_t(255); }
+  std::map<size_t, size_t> free_;
 };
 
 hipError_t job_footprint(const bc_amd_job &job, request *r) {
@@ -77,24 +133,20 @@ class server {
 public:
   server(int device, size_t budget, unsigned slots) : device_(device), budget_(budget), slots_(slots) {}
 
-  // Each slot has its own stream and memory pool: a pool shared by the slots
-  // would hand memory freed on one slot's stream to another only after making
-  // that stream wait for the first, which serialises the batches.
   hipError_t start() {
+    BC_TRY(hipMalloc(&workspace_, budget_));
+    heap_.reset(budget_);
     hipMemPoolProps props{};
     props.allocType = hipMemAllocationTypePinned;
     props.location.type = hipMemLocationTypeDevice;
     props.location.id = device_;
+    BC_TRY(hipMemPoolCreate(&pool_, &props));
     streams_.resize(slots_, nullptr);
     events_.resize(slots_, nullptr);
-    pools_.resize(slots_, nullptr);
     busy_.assign(slots_, false);
     for (unsigned i = 0; i < slots_; i++) {
       BC_TRY(hipStreamCreateWithFlags(&streams_[i], hipStreamNonBlocking));
       BC_TRY(hipEventCreateWithFlags(&events_[i], hipEventDisableTiming));
-      BC_TRY(hipMemPoolCreate(&pools_[i], &props));
-      uint64_t threshold = UINT64_MAX; // keep freed memory reserved for the slot's next batches
-      BC_TRY(hipMemPoolSetAttribute(pools_[i], hipMemPoolAttrReleaseThreshold, &threshold));
     }
     thread_ = std::thread([this] { loop(); });
     return hipSuccess;
@@ -115,9 +167,10 @@ public:
     for (hipEvent_t e : events_)
       if (e)
         (void)hipEventDestroy(e);
-    for (hipMemPool_t p : pools_)
-      if (p)
-        (void)hipMemPoolDestroy(p);
+    if (pool_)
+      (void)hipMemPoolDestroy(pool_);
+    if (workspace_)
+      (void)hipFree(workspace_);
     for (auto &kv : all_)
       delete kv.second;
   }
@@ -159,6 +212,44 @@ public:
     return hipSuccess;
   }
 
+  struct work {
+    std::vector<msm_job> msm;
+    std::vector<ntt_job> ntt;
+    std::vector<const bc_amd_pn_job *> pn;
+  };
+
+  static work collect(const batch &b) {
+    work w;
+    for (const request *r : b.reqs) {
+      if (r->kind == BC_AMD_JOB_MSM)
+        w.msm.push_back(r->msm);
+      else if (r->kind == BC_AMD_JOB_NTT)
+        w.ntt.push_back(r->ntt);
+      else
+        w.pn.push_back(&r->pn);
+    }
+    return w;
+  }
+
+  static hipError_t record(const work &w, arena &ar) {
+    BC_TRY(msm_run(w.msm, ar));
+    BC_TRY(ntt_run(w.ntt, ar));
+    for (const bc_amd_pn_job *p : w.pn)
+      BC_TRY(pn_generate_device(p->indexes, static_cast<const fe *>(p->scalars), static_cast<fe *>(p->target), p->columns_count, p->log_rows_count, ar));
+    return hipSuccess;
+  }
+
+  // Exact workspace bytes of a batch, by a sizing capture.  Twiddle tables are
+  // grown first, outside the capture.
+  static hipError_t size_of(const work &w, hipStream_t s, size_t *bytes) {
+    unsigned max_log = 0;
+    for (const ntt_job &j : w.ntt)
+      max_log = std::max(max_log, j.log_n);
+    if (max_log)
+      BC_TRY(ntt_prepare(max_log, s, nullptr));
+    return measure_scratch(s, [&](arena &ar) { return record(w, ar); }, bytes);
+  }
+
 private:
   // Longest FIFO prefix that fits the free memory and the per-launch limits,
   // and takes no more than its share of the waiting jobs when several slots
@@ -196,28 +287,35 @@ private:
     return !b.reqs.empty();
   }
 
-  hipError_t launch(batch &b) {
+  // Sizes the batch, places it in the workspace and launches it on its slot's
+  // stream.  When the region does not fit while other batches hold parts of
+  // the workspace, nothing is launched and *deferred is set; a batch that does
+  // not fit the empty workspace runs alone from the pool.
+  hipError_t place_and_launch(batch &b, bool *deferred) {
+    *deferred = false;
+    const work w = collect(b);
     hipStream_t s = streams_[b.slot];
-    std::vector<msm_job> msm;
-    std::vector<ntt_job> ntt;
-    std::vector<const bc_amd_pn_job *> pn;
+    size_t need = 0;
+    BC_TRY(size_of(w, s, &need));
+    const bool placed = heap_.take(need, &b.region);
+    if (!placed && !inflight_.empty()) {
+      *deferred = true;
+      return hipSuccess;
+    }
     const uint64_t t = now_ns();
     for (request *r : b.reqs) {
       r->stats.start_ns = t;
       r->stats.batch_jobs = (unsigned)b.reqs.size();
-      if (r->kind == BC_AMD_JOB_MSM)
-        msm.push_back(r->msm);
-      else if (r->kind == BC_AMD_JOB_NTT)
-        ntt.push_back(r->ntt);
-      else
-        pn.push_back(&r->pn);
+      r->stats.slot = b.slot;
     }
-    arena ar = arena::pool_mode(pools_[b.slot], s);
-    BC_TRY(msm_run(msm, ar));
-    BC_TRY(ntt_run(ntt, ar));
-    for (const bc_amd_pn_job *p : pn)
-      BC_TRY(pn_generate_device(p->indexes, static_cast<const fe *>(p->scalars), static_cast<fe *>(p->target), p->columns_count, p->log_rows_count, ar));
-    return hipEventRecord(events_[b.slot], s);
+    b.region_bytes = placed ? need : 0;
+    arena ar = placed ? arena::bump_mode(static_cast<char *>(workspace_) + b.region, need, s) : arena::pool_mode(pool_, s);
+    BC_TRY(record(w, ar));
+    BC_TRY(hipEventRecord(events_[b.slot], s));
+    const uint64_t enqueued = now_ns();
+    for (request *r : b.reqs)
+      r->stats.enqueued_ns = enqueued;
+    return hipSuccess;
   }
 
   void finish(batch &b, hipError_t status) {
@@ -227,6 +325,8 @@ private:
       r->stats.status = as_bc(status);
       r->done = true;
     }
+    if (b.region_bytes)
+      heap_.give(b.region, b.region_bytes);
     used_ -= b.bytes;
     busy_[b.slot] = false;
   }
@@ -234,6 +334,7 @@ private:
   void loop() {
     (void)hipSetDevice(device_);
     std::unique_lock<std::mutex> lk(m_);
+    bool deferred = false; // a batch waits for workspace that a running batch holds
     for (;;) {
       bool progress = false;
       for (size_t i = 0; i < inflight_.size();) {
@@ -248,8 +349,10 @@ private:
         inflight_.erase(inflight_.begin() + i);
         progress = true;
       }
-      if (progress)
+      if (progress) {
+        deferred = false;
         cv_done_.notify_all();
+      }
       int slot = -1;
       unsigned free_slots = 0;
       for (unsigned i = 0; i < slots_; i++)
@@ -258,18 +361,25 @@ private:
             slot = (int)i;
           free_slots++;
         }
-      if (slot >= 0 && !pending_.empty()) {
+      if (slot >= 0 && !pending_.empty() && !deferred) {
         batch b;
         b.slot = (unsigned)slot;
         if (take_batch(b, free_slots)) {
           used_ += b.bytes;
           busy_[b.slot] = true;
           lk.unlock();
-          hipError_t e = launch(b);
+          bool later = false;
+          hipError_t e = place_and_launch(b, &later);
           if (e != hipSuccess)
             (void)hipStreamSynchronize(streams_[b.slot]); // work already enqueued must end before its memory is reused
           lk.lock();
-          if (e != hipSuccess) {
+          if (later) {
+            for (auto it = b.reqs.rbegin(); it != b.reqs.rend(); ++it)
+              pending_.push_front(*it);
+            used_ -= b.bytes;
+            busy_[b.slot] = false;
+            deferred = true;
+          } else if (e != hipSuccess) {
             finish(b, e);
             cv_done_.notify_all();
           } else {
@@ -291,7 +401,9 @@ private:
   size_t budget_;
   unsigned slots_;
   size_t used_ = 0;
-  std::vector<hipMemPool_t> pools_;
+  void *workspace_ = nullptr;
+  region_heap heap_;
+  hipMemPool_t pool_ = nullptr;
   std::vector<hipStream_t> streams_;
   std::vector<hipEvent_t> events_;
   std::vector<bool> busy_;
@@ -320,6 +432,21 @@ bc_error bc_amd_job_footprint(const bc_amd_job *job, size_t *bytes) {
   BC_TRY_BC(job_footprint(*job, &r));
   *bytes = r.footprint;
   return bc_success;
+}
+
+bc_error bc_amd_job_workspace_bytes(const bc_amd_job *job, size_t *bytes) {
+  if (!job || !bytes)
+    return bc_error_invalid_value;
+  request r;
+  BC_TRY_BC(job_footprint(*job, &r));
+  batch b;
+  b.reqs.push_back(&r);
+  const auto w = server::collect(b);
+  hipStream_t s = nullptr;
+  BC_TRY_BC(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  const hipError_t e = server::size_of(w, s, bytes);
+  (void)hipStreamDestroy(s);
+  return as_bc(e);
 }
 
 bc_error bc_amd_server_create(bc_amd_server *server_out, size_t hbm_budget_bytes, unsigned max_in_flight) {

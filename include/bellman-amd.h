@@ -134,6 +134,8 @@ typedef struct {
   unsigned long long done_ns;   /* batch completion observed */
   unsigned batch_jobs;          /* jobs of the batch that ran this one */
   bc_error status;
+  unsigned long long enqueued_ns; /* the batch's last launch returned to the host */
+  unsigned slot;                  /* the server slot (stream) that ran the batch */
 } bc_amd_job_stats;
 
 typedef struct {
@@ -142,9 +144,14 @@ typedef struct {
 
 /* Temporary device memory the job needs while it runs (upper bound). */
 bc_error bc_amd_job_footprint(const bc_amd_job *job, size_t *bytes);
+/* Exact temporary device memory of the job run alone (the high-water mark of
+ * its launch sequence, recorded on an internal stream and not run); at most
+ * bc_amd_job_footprint.  Nothing is allocated or launched. */
+bc_error bc_amd_job_workspace_bytes(const bc_amd_job *job, size_t *bytes);
 
-/* Uses the current device; hbm_budget_bytes bounds the summed footprints of
- * running batches (a single job larger than the budget runs alone). */
+/* Uses the current device and allocates hbm_budget_bytes of it at once as the
+ * server's workspace; the summed footprints of running batches stay within it
+ * (a single job larger than the budget runs alone, from a memory pool). */
 bc_error bc_amd_server_create(bc_amd_server *server, size_t hbm_budget_bytes, unsigned max_in_flight);
 /* The job's buffers must stay valid until the job completes. */
 bc_error bc_amd_server_submit(bc_amd_server server, const bc_amd_job *job, unsigned long long *ticket);

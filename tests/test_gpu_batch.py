@@ -261,7 +261,8 @@ def test_server_results_same_alone_and_in_mix(bellman, oracle, shared_bases):
         assert all(s.status == 0 for s in stats)
         assert max(s.batch_jobs for s in stats) > 1  # jobs really shared batches
         for s in stats:
-            assert s.submit_ns <= s.start_ns <= s.done_ns
+            assert s.submit_ns <= s.start_ns <= s.enqueued_ns <= s.done_ns
+            assert s.slot < 2
         for (job, exp), a in zip(jobs, alone):
             got = _output(dev, job)
             assert np.array_equal(got, a), exp[0]
@@ -277,6 +278,52 @@ def test_server_results_same_alone_and_in_mix(bellman, oracle, shared_bases):
                 assert np.array_equal(got, pn(dev, idx, ks, cols, lr))
     finally:
         srv.close()
+
+
+def test_footprint_bounds_exact_workspace(bellman, oracle, shared_bases):
+    """bc_amd_job_footprint (the admission estimate, and the block a plain call
+    takes from its pool) is at least the exact workspace of the job run alone;
+    for MSM and NTT jobs that workspace is also what a single-job plan holds."""
+    _, dev = bellman
+    db, _, _ = shared_bases
+    k = dev.upload(oracle.msm_scalars(0xBC000050, 1 << 12))
+    r = dev.malloc(254 * 96)
+    x = dev.upload(oracle.random_fr(0xBC000051, 1 << 12))
+    y = dev.malloc(32 << 12)
+    idx = dev.upload(pn_cycles(4, 10, 0xBC000052))
+    ks = dev.upload(oracle.random_fr(0xBC000053, 4))
+    t = dev.malloc(32 * (4 << 10))
+    cases = [("msm", B.msm_job(db, k, r, L, c, piece)) for L, c, piece in [(12, 13, 64), (12, 4, 1), (10, 16, 8), (0, 3, 32), (12, 9, 128)]]
+    cases += [("ntt", B.ntt_job(x, y, L, br, inv, le, co)) for L, br, inv, le, co in [(12, False, False, 0, 0), (10, True, True, 2, 3), (12, False, True, 0, 0)]]
+    cases += [("pn", B.bc_amd_pn_job(idx, ks, t, cols, lr)) for cols, lr in [(4, 10), (3, 10), (4, 6)]]
+    try:
+        for kind, j in cases:
+            job = B.bc_amd_job()
+            exact, fp = ctypes.c_size_t(), ctypes.c_size_t()
+            if kind == "msm":
+                job.kind, job.msm = B.BC_AMD_JOB_MSM, j
+            elif kind == "ntt":
+                job.kind, job.ntt = B.BC_AMD_JOB_NTT, j
+            else:
+                job.kind, job.pn = B.BC_AMD_JOB_PN, j
+            B.call("bc_amd_job_workspace_bytes", ctypes.byref(job), ctypes.byref(exact))
+            B.call("bc_amd_job_footprint", ctypes.byref(job), ctypes.byref(fp))
+            assert fp.value >= exact.value, (kind, fp.value, exact.value)
+            if kind == "pn":
+                continue
+            plan, ws = B.bc_amd_plan(), ctypes.c_size_t()
+            if kind == "msm":
+                B.call("bc_amd_plan_create", ctypes.byref(plan), B.array_of(B.bc_amd_msm_job, [j]), ctypes.c_uint(1), None, ctypes.c_uint(0))
+            else:
+                B.call("bc_amd_plan_create", ctypes.byref(plan), None, ctypes.c_uint(0), B.array_of(B.bc_amd_ntt_job, [j]), ctypes.c_uint(1))
+            try:
+                B.call("bc_amd_plan_workspace_bytes", plan, ctypes.byref(ws))
+            finally:
+                B.call("bc_amd_plan_destroy", plan)
+            assert ws.value == exact.value, (kind, ws.value, exact.value)
+    finally:
+        for p in (k, r, x, y, idx, ks, t):
+            dev.free(p)
 
 
 def test_server_tight_budget_runs_everything(bellman, oracle, shared_bases):

@@ -15,17 +15,23 @@ def B():
     return B
 
 
-def test_msm_tuning_defaults_are_the_reference_windows(B):
-    # src/msm.cu:12-37 window widths; piece 32
-    expected = {12: 12, 14: 13, 16: 15, 19: 15, 20: 16, 22: 17, 23: 19, 24: 20, 26: 22}
-    saved = {}
-    for L, c in expected.items():
-        t = B.bc_amd_msm_tuning()
-        B.call("bc_amd_msm_get_tuning", ctypes.c_uint(L), ctypes.byref(t))
-        saved[L] = (t.window_bits, t.piece)
-    # the table may have been tuned by an earlier test in this process; only check validity then
-    for L, (c, piece) in saved.items():
-        assert 3 <= c <= 24 and piece >= 1
+def test_msm_tuning_builtin_table(B):
+    """A fresh process starts from the built-in table: measured (window, piece)
+    for 2^12..2^24, the plain call's window width (src/msm.cu:12-37) above."""
+    import subprocess
+    import sys
+
+    code = ("import ctypes, bellman_amd as B\n"
+            "for L in range(0, 29):\n"
+            "    t = B.bc_amd_msm_tuning(); B.call('bc_amd_msm_get_tuning', ctypes.c_uint(L), ctypes.byref(t)); print(L, t.window_bits, t.piece)\n")
+    pkg = os.path.dirname(os.path.dirname(B.__file__))
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, env=dict(os.environ, PYTHONPATH=pkg))
+    table = {int(a): (int(b), int(c)) for a, b, c in (line.split() for line in out.stdout.splitlines())}
+    assert table[12] == (5, 32) and table[16] == (11, 16) and table[20] == (13, 16) and table[24] == (17, 32)
+    assert table[25] == (20, 32) and table[26] == (22, 32) and table[28] == (22, 32)
+    for L, (c, piece) in table.items():
+        assert 3 <= c <= 22 and piece in (16, 32)
+        assert (254 - 1) // c + 1 <= 85  # windows of at least 3 bits
 
 
 def test_tuning_validation(B):
