@@ -141,7 +141,7 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 
 // 1. signed digits.  Entry (w, i) of a job gets the window-local key d - 1
 // (2^(c-1), after every bucket, for d = 0) and value = sign << 31 | i.
-__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint32_t total, uint32_t *__restrict__ keys, uint32_t *__restrict__ values) {
+__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint32_t total, uint64_t *__restrict__ entries) {
   using F = FrParams;
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total)
@@ -165,29 +165,28 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
       sign ^= SIGN;
     }
     const uint64_t e = jd.entry_begin + (uint64_t)w * n + i;
-    keys[e] = d ? d - 1 : top;
-    values[e] = sign | i;
+    entries[e] = (uint64_t)(sign | i) << 32 | (d ? d - 1 : top); // key low, point reference high
   }
 }
 
-// 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted arrays
+// 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted entries
 // (window w of a job is the segment [entry_begin + w n, entry_begin + (w+1) n))
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint32_t *__restrict__ keys, uint32_t entries,
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t entries,
                                                                 uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
   const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= entries)
     return;
   const job_desc &jd = jobs.job[find_job(jobs, e, by_entry())];
-  const uint32_t k = keys[e];
+  const uint32_t k = (uint32_t)sorted[e];
   const uint32_t top = 1u << (jd.c - 1);
   if (k == top)
     return; // zero digits
   const uint32_t local = e - (uint32_t)jd.entry_begin, w = local >> jd.log_n;
   const uint32_t seg_begin = (uint32_t)jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
   const uint32_t b = jd.bucket_begin + w * top + k;
-  if (e == seg_begin || keys[e - 1] != k)
+  if (e == seg_begin || (uint32_t)sorted[e - 1] != k)
     start[b] = e;
-  if (e + 1 == seg_end || keys[e + 1] != k)
+  if (e + 1 == seg_end || (uint32_t)sorted[e + 1] != k)
     end[b] = e + 1;
 }
 
@@ -233,7 +232,7 @@ __global__ __launch_bounds__(256) void piece_map_big_kernel(const uint32_t *piec
 }
 
 // 3. one thread accumulates one piece (<= piece points of one bucket)
-__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint32_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
+__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
                                                                const uint32_t *__restrict__ counts, const uint32_t *__restrict__ piece_offset,
                                                                const uint32_t *__restrict__ piece_bucket, const uint32_t *__restrict__ total,
                                                                g1_xyzz *__restrict__ partials) {
@@ -251,10 +250,10 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   g1_xyzz acc = xyzz_infinity();
   if (begin < end) {
     // the next point's loads are issued before the current addition
-    uint32_t ref = sorted[begin];
+    uint32_t ref = (uint32_t)(sorted[begin] >> 32);
     g1_affine pt = load_affine(bases + (ref & ~SIGN));
     for (uint32_t e = begin; e < end; e++) {
-      const uint32_t ref_next = e + 1 < end ? sorted[e + 1] : ref;
+      const uint32_t ref_next = e + 1 < end ? (uint32_t)(sorted[e + 1] >> 32) : ref;
       const g1_affine next = load_affine(bases + (ref_next & ~SIGN));
       if (ref & SIGN)
         pt.y = neg2<Fq>(pt.y);
@@ -579,16 +578,15 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   const uint32_t nb = (uint32_t)nb64;
 
   // 1-2. digits, sort, bucket runs
-  uint32_t *offsets = nullptr, *counts = nullptr, *sorted = nullptr;
+  uint32_t *offsets = nullptr, *counts = nullptr;
+  uint64_t *sorted = nullptr; // (key, point reference) pairs, sorted by key within each window
   BC_TRY(ar.get(&offsets, nb));
   BC_TRY(ar.get(&counts, nb));
   BC_TRY(ar.get(&sorted, entries));
   {
-    uint32_t *keys = nullptr, *vals = nullptr, *sorted_keys = nullptr;
-    BC_TRY(ar.get(&keys, entries));
-    BC_TRY(ar.get(&vals, entries));
-    BC_TRY(ar.get(&sorted_keys, entries));
-    digits_kernel<<<grid_for(scalars_total, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, scalars_total, keys, vals);
+    uint64_t *digits = nullptr;
+    BC_TRY(ar.get(&digits, entries));
+    digits_kernel<<<grid_for(scalars_total, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, scalars_total, digits);
     BC_LAUNCH_CHECK();
     seg_group groups[MSM_MAX_JOBS];
     unsigned key_bits = 0; // local keys are <= 2^(c-1)
@@ -596,14 +594,12 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
       groups[j] = {jt.job[j].entry_begin, jt.job[j].n, jt.job[j].W};
       key_bits = std::max(key_bits, jt.job[j].c);
     }
-    BC_TRY(segmented_sort_pairs(keys, sorted_keys, vals, sorted, groups, count, key_bits, ar));
+    BC_TRY(segmented_sort_kv(digits, sorted, groups, count, key_bits, ar));
+    BC_TRY(ar.put(digits));
     BC_TRY(fill_u32(offsets, nb, 0, s));
     BC_TRY(fill_u32(counts, nb, 0, s));
-    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, sorted_keys, (uint32_t)entries, offsets, counts);
+    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, sorted, (uint32_t)entries, offsets, counts);
     BC_LAUNCH_CHECK();
-    BC_TRY(ar.put(keys));
-    BC_TRY(ar.put(vals));
-    BC_TRY(ar.put(sorted_keys));
   }
 
   // 3. pieces and accumulation; meta = {piece total, heavy count, heavy chunk total, big count}

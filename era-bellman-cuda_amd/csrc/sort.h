@@ -31,4 +31,8 @@ struct seg_group {
 hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
                                 unsigned group_count, unsigned key_bits, arena &ar);
 
+// The same on u64 entries whose low 32 bits are the key (the high word is the
+// value): one load and one store per entry and pass.
+hipError_t segmented_sort_kv(const uint64_t *in, uint64_t *out, const seg_group *groups, unsigned group_count, unsigned key_bits, arena &ar);
+
 } // namespace bc

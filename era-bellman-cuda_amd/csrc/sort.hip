@@ -75,7 +75,9 @@ __device__ __forceinline__ ss_tile ss_locate(const ss_table &t, unsigned digits)
   return r;
 }
 
-__global__ __launch_bounds__(SS_BLOCK) void ss_histogram_kernel(const ss_table t, const uint32_t *__restrict__ keys, unsigned shift, unsigned bits,
+// Entries are u32 keys, or u64 key-value pairs with the key in the low word.
+template <class E>
+__global__ __launch_bounds__(SS_BLOCK) void ss_histogram_kernel(const ss_table t, const E *__restrict__ keys, unsigned shift, unsigned bits,
                                                                  uint32_t *__restrict__ table) {
   __shared__ uint32_t hist[SS_MAX_DIGITS];
   const unsigned digits = 1u << bits;
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_histogram_kernel(const ss_table t
   const ss_tile tl = ss_locate(t, digits);
   const uint32_t mask = digits - 1;
   for (unsigned i = threadIdx.x; i < tl.len; i += SS_BLOCK)
-    atomicAdd(&hist[(keys[tl.start + i] >> shift) & mask], 1u);
+    atomicAdd(&hist[((uint32_t)keys[tl.start + i] >> shift) & mask], 1u);
   __syncthreads();
   for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
     table[tl.slot0 + (uint64_t)d * tl.stride] = hist[d];
@@ -147,53 +149,94 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kernel(const ss_table t, 
   }
 }
 
-} // namespace
+// Key-value entries (key in the low word), ranked as above: one 8-byte load and
+// one scattered 8-byte store per entry instead of two of each.  On MI355X this
+// pass takes 1.75 ms for 218 M entries against 2.33 ms with separate key and
+// value arrays (the scattered stores, not their bytes, bound it).  Staging the
+// tile in LDS in digit order to coalesce the stores was slower (2.85 ms): its
+// 72 KiB of LDS leaves two blocks per CU.
+__global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kv_kernel(const ss_table t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                                                 unsigned shift, unsigned bits, const uint32_t *__restrict__ table) {
+  __shared__ uint32_t wave_hist[SS_WAVES][SS_MAX_DIGITS];
+  const unsigned digits = 1u << bits;
+  const uint32_t mask = digits - 1;
+  const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
+    for (unsigned w = 0; w < SS_WAVES; w++)
+      wave_hist[w][d] = 0;
+  __syncthreads();
+  const ss_tile tl = ss_locate(t, digits);
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  uint64_t item[SS_ITEMS];
+  uint32_t rank[SS_ITEMS];
+#pragma unroll
+  for (unsigned s = 0; s < SS_ITEMS; s++) {
+    const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
+    const bool valid = i < tl.len;
+    item[s] = valid ? in[tl.start + i] : 0ull;
+    const uint32_t d = ((uint32_t)item[s] >> shift) & mask;
+    uint64_t peers = __ballot(valid);
+    for (unsigned b = 0; b < bits; b++) {
+      const uint64_t m = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const unsigned before = __popcll(peers & lt_mask);
+    const uint32_t base = valid ? wave_hist[wave][d] : 0u;
+    rank[s] = base + before;
+    if (valid && before == 0)
+      wave_hist[wave][d] = base + (uint32_t)__popcll(peers);
+  }
+  __syncthreads();
+  for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK) {
+    uint32_t run = table[tl.slot0 + (uint64_t)d * tl.stride];
+    for (unsigned w = 0; w < SS_WAVES; w++) {
+      const uint32_t c = wave_hist[w][d];
+      wave_hist[w][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (unsigned s = 0; s < SS_ITEMS; s++) {
+    const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
+    if (i < tl.len)
+      out[wave_hist[wave][((uint32_t)item[s] >> shift) & mask] + rank[s]] = item[s];
+  }
+}
 
-hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
-                                unsigned group_count, unsigned key_bits, arena &ar) {
-  hipStream_t stream = ar.stream();
-  if (group_count == 0)
-    return hipSuccess;
+// Pass structure shared by both entry points: balanced digit widths (wider
+// first) and each pass's tile table.
+struct ss_plan {
+  unsigned passes = 0, widths[4] = {}, shifts[4] = {};
+  uint64_t n = 0, table_entries = 0;
+  ss_table t[4];
+  uint64_t slots[4] = {};
+};
+
+hipError_t ss_make_plan(const seg_group *groups, unsigned group_count, unsigned key_bits, ss_plan &pl) {
   if (group_count > SEG_MAX_GROUPS || key_bits > 32)
     return hipErrorInvalidValue;
-  uint64_t n = 0, tiles = 0;
+  uint64_t tiles = 0;
   for (unsigned g = 0; g < group_count; g++) {
-    if (groups[g].begin != n || groups[g].length == 0)
+    if (groups[g].begin != pl.n || groups[g].length == 0)
       return hipErrorInvalidValue; // contiguous from entry 0
-    n += (uint64_t)groups[g].length * groups[g].count;
+    pl.n += (uint64_t)groups[g].length * groups[g].count;
     tiles += (uint64_t)groups[g].count * ((groups[g].length + SS_TILE - 1) / SS_TILE);
   }
-  if (n == 0)
-    return hipSuccess;
-  if (n >= (1ull << 32) || tiles >= (1ull << 31))
+  if (pl.n >= (1ull << 32) || tiles >= (1ull << 31))
     return hipErrorInvalidValue;
-  const bool with_values = values_in != nullptr;
-  if (key_bits == 0) {
-    BC_TRY(hipMemcpyAsync(keys_out, keys_in, n * 4, hipMemcpyDeviceToDevice, stream));
-    if (with_values)
-      BC_TRY(hipMemcpyAsync(values_out, values_in, n * 4, hipMemcpyDeviceToDevice, stream));
+  if (pl.n == 0 || key_bits == 0)
     return hipSuccess;
+  pl.passes = (key_bits + SS_MAX_BITS - 1) / SS_MAX_BITS;
+  for (unsigned p = 0, shift = 0; p < pl.passes; p++) {
+    pl.widths[p] = key_bits / pl.passes + (p < key_bits % pl.passes ? 1 : 0);
+    pl.shifts[p] = shift;
+    shift += pl.widths[p];
   }
-  const unsigned passes = (key_bits + SS_MAX_BITS - 1) / SS_MAX_BITS;
-  // balanced digit widths, wider first
-  unsigned widths[4], shifts[4];
-  for (unsigned p = 0, shift = 0; p < passes; p++) {
-    widths[p] = key_bits / passes + (p < key_bits % passes ? 1 : 0);
-    shifts[p] = shift;
-    shift += widths[p];
-  }
-  uint64_t table_entries = 0;
-  for (unsigned g = 0; g < group_count; g++)
-    table_entries += (uint64_t)groups[g].count * (1u << widths[0]) * ((groups[g].length + SS_TILE - 1) / SS_TILE);
-  uint32_t *table = nullptr, *tmp_k = nullptr, *tmp_v = nullptr;
-  BC_TRY(ar.get(&table, table_entries));
-  BC_TRY(ar.get(&tmp_k, n));
-  if (with_values)
-    BC_TRY(ar.get(&tmp_v, n));
-  const uint32_t *src_k = keys_in, *src_v = values_in;
-  for (unsigned p = 0; p < passes; p++) {
-    const unsigned digits = 1u << widths[p];
-    ss_table t{};
+  for (unsigned p = 0; p < pl.passes; p++) {
+    const unsigned digits = 1u << pl.widths[p];
+    ss_table &t = pl.t[p];
+    t = ss_table{};
     t.count = group_count;
     uint64_t slots = 0;
     uint32_t tile = 0;
@@ -209,25 +252,83 @@ hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, con
       slots += (uint64_t)sg.count * digits * sg.tiles_per_seg;
     }
     t.tiles = tile;
-    const bool to_out = ((passes - 1 - p) % 2) == 0; // the last pass lands in *_out
+    pl.slots[p] = slots;
+  }
+  pl.table_entries = pl.slots[0]; // the widest pass comes first
+  return hipSuccess;
+}
+
+} // namespace
+
+hipError_t segmented_sort_kv(const uint64_t *in, uint64_t *out, const seg_group *groups, unsigned group_count, unsigned key_bits, arena &ar) {
+  hipStream_t stream = ar.stream();
+  if (group_count == 0)
+    return hipSuccess;
+  ss_plan pl;
+  BC_TRY(ss_make_plan(groups, group_count, key_bits, pl));
+  if (pl.n == 0)
+    return hipSuccess;
+  if (pl.passes == 0)
+    return hipMemcpyAsync(out, in, pl.n * 8, hipMemcpyDeviceToDevice, stream);
+  uint32_t *table = nullptr;
+  uint64_t *tmp = nullptr;
+  BC_TRY(ar.get(&table, pl.table_entries));
+  BC_TRY(ar.get(&tmp, pl.n));
+  const uint64_t *src = in;
+  for (unsigned p = 0; p < pl.passes; p++) {
+    uint64_t *dst = ((pl.passes - 1 - p) % 2) == 0 ? out : tmp; // the last pass lands in `out`
+    ss_histogram_kernel<uint64_t><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, pl.shifts[p], pl.widths[p], table);
+    BC_LAUNCH_CHECK();
+    BC_TRY(scan_u32_exclusive(table, table, pl.slots[p], ar));
+    ss_scatter_kv_kernel<<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, dst, pl.shifts[p], pl.widths[p], table);
+    BC_LAUNCH_CHECK();
+    src = dst;
+  }
+  BC_TRY(ar.put(tmp));
+  return ar.put(table);
+}
+
+hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
+                                unsigned group_count, unsigned key_bits, arena &ar) {
+  hipStream_t stream = ar.stream();
+  if (group_count == 0)
+    return hipSuccess;
+  ss_plan pl;
+  BC_TRY(ss_make_plan(groups, group_count, key_bits, pl));
+  if (pl.n == 0)
+    return hipSuccess;
+  const bool with_values = values_in != nullptr;
+  if (pl.passes == 0) {
+    BC_TRY(hipMemcpyAsync(keys_out, keys_in, pl.n * 4, hipMemcpyDeviceToDevice, stream));
+    if (with_values)
+      BC_TRY(hipMemcpyAsync(values_out, values_in, pl.n * 4, hipMemcpyDeviceToDevice, stream));
+    return hipSuccess;
+  }
+  uint32_t *table = nullptr, *tmp_k = nullptr, *tmp_v = nullptr;
+  BC_TRY(ar.get(&table, pl.table_entries));
+  BC_TRY(ar.get(&tmp_k, pl.n));
+  if (with_values)
+    BC_TRY(ar.get(&tmp_v, pl.n));
+  const uint32_t *src_k = keys_in, *src_v = values_in;
+  for (unsigned p = 0; p < pl.passes; p++) {
+    const bool to_out = ((pl.passes - 1 - p) % 2) == 0; // the last pass lands in *_out
     uint32_t *dst_k = to_out ? keys_out : tmp_k;
     uint32_t *dst_v = to_out ? values_out : tmp_v;
-    ss_histogram_kernel<<<tile, SS_BLOCK, 0, stream>>>(t, src_k, shifts[p], widths[p], table);
+    ss_histogram_kernel<uint32_t><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src_k, pl.shifts[p], pl.widths[p], table);
     BC_LAUNCH_CHECK();
-    BC_TRY(scan_u32_exclusive(table, table, slots, ar));
+    BC_TRY(scan_u32_exclusive(table, table, pl.slots[p], ar));
     if (with_values)
-      ss_scatter_kernel<true><<<tile, SS_BLOCK, 0, stream>>>(t, src_k, dst_k, src_v, dst_v, shifts[p], widths[p], table);
+      ss_scatter_kernel<true><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src_k, dst_k, src_v, dst_v, pl.shifts[p], pl.widths[p], table);
     else
-      ss_scatter_kernel<false><<<tile, SS_BLOCK, 0, stream>>>(t, src_k, dst_k, nullptr, nullptr, shifts[p], widths[p], table);
+      ss_scatter_kernel<false><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src_k, dst_k, nullptr, nullptr, pl.shifts[p], pl.widths[p], table);
     BC_LAUNCH_CHECK();
     src_k = dst_k;
     src_v = dst_v;
   }
-  BC_TRY(ar.put(table));
-  BC_TRY(ar.put(tmp_k));
   if (with_values)
     BC_TRY(ar.put(tmp_v));
-  return hipSuccess;
+  BC_TRY(ar.put(tmp_k));
+  return ar.put(table);
 }
 
 // One-segment sorts on key bits [0, key_bits).
