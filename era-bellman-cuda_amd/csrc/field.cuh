@@ -81,45 +81,42 @@ BC_HD bool fe_eq(const fe &a, const fe &b) {
   return o == 0;
 }
 
+// Limb add/subtract as explicit carry chains (clang's __builtin_addc/subc):
+// on gfx950 they lower to v_add_co/v_addc_co (v_sub_co/v_subb_co) pairs, 8
+// VALU instructions per 256-bit add, where the 64-bit-accumulator idiom
+// compiled to 64-bit shift-adds and moves (2-3x the instructions).
+
 // r = a - b over 256 bits; returns the borrow (0/1)
 BC_HD uint32_t sub_limbs(fe &r, const fe &a, const fe &b) {
-  uint64_t t = 0;
+  unsigned c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    t = (uint64_t)a.v[i] - b.v[i] - (uint32_t)(t >> 63);
-    r.v[i] = (uint32_t)t;
-  }
-  return (uint32_t)(t >> 63);
+  for (int i = 0; i < 8; i++)
+    r.v[i] = __builtin_subc(a.v[i], b.v[i], c, &c);
+  return c;
 }
 
 // r = a + b over 256 bits; returns the carry (0/1)
 BC_HD uint32_t add_limbs(fe &r, const fe &a, const fe &b) {
-  uint64_t t = 0;
+  unsigned c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    t = (uint64_t)a.v[i] + b.v[i] + (uint32_t)(t >> 32);
-    r.v[i] = (uint32_t)t;
-  }
-  return (uint32_t)(t >> 32);
+  for (int i = 0; i < 8; i++)
+    r.v[i] = __builtin_addc(a.v[i], b.v[i], c, &c);
+  return c;
 }
 
 BC_HD uint32_t sub_const(fe &r, const fe &a, const uint32_t (&b)[8]) {
-  uint64_t t = 0;
+  unsigned c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    t = (uint64_t)a.v[i] - b[i] - (uint32_t)(t >> 63);
-    r.v[i] = (uint32_t)t;
-  }
-  return (uint32_t)(t >> 63);
+  for (int i = 0; i < 8; i++)
+    r.v[i] = __builtin_subc(a.v[i], b[i], c, &c);
+  return c;
 }
 
 BC_HD void add_const(fe &r, const fe &a, const uint32_t (&b)[8]) {
-  uint64_t t = 0;
+  unsigned c = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    t = (uint64_t)a.v[i] + b[i] + (uint32_t)(t >> 32);
-    r.v[i] = (uint32_t)t;
-  }
+  for (int i = 0; i < 8; i++)
+    r.v[i] = __builtin_addc(a.v[i], b[i], c, &c);
 }
 
 // a < m (or < 2m for lazy) -> subtract m once if a >= m
@@ -143,10 +140,10 @@ template <class F> BC_HD fe add(const fe &a, const fe &b) {
 }
 
 template <class F> BC_HD fe sub(const fe &a, const fe &b) {
-  fe d;
-  if (sub_limbs(d, a, b))
-    add_const(d, d, F::m);
-  return d;
+  fe d, e;
+  const uint32_t borrow = sub_limbs(d, a, b);
+  add_const(e, d, F::m);
+  return borrow ? e : d;
 }
 
 template <class F> BC_HD fe neg(const fe &a) {
@@ -164,10 +161,10 @@ template <class F> BC_HD fe add2(const fe &a, const fe &b) {
 }
 
 template <class F> BC_HD fe sub2(const fe &a, const fe &b) {
-  fe d;
-  if (sub_limbs(d, a, b))
-    add_const(d, d, F::m2);
-  return d;
+  fe d, e;
+  const uint32_t borrow = sub_limbs(d, a, b);
+  add_const(e, d, F::m2);
+  return borrow ? e : d;
 }
 
 template <class F> BC_HD fe dbl2(const fe &a) { return add2<F>(a, a); }
