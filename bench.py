@@ -647,7 +647,7 @@ def main():
         ntt_ms, inv_ms, lde_ms, ntt_ok = headline_ntt(lab, args.log_ntt, args.steps, args.warmup)
         log("ntt 2^%d: %.3f ms, inverse %.3f ms, lde x4 %.3f ms, round trip ok=%s" % (args.log_ntt, ntt_ms, inv_ms, lde_ms, ntt_ok))
         n = 1 << args.log_ntt
-        launches = (args.log_ntt + 11) // 12
+        launches = (args.log_ntt + 9) // 10  # passes of at most 10 stages
         per_launch_ms = ntt_ms / launches
         achieved = 64.0 * n / (per_launch_ms * 1e-3) / 1e9
         out.update({"ntt_ms": round(ntt_ms, 3), "ntt_inverse_ms": round(inv_ms, 3), "lde_4_cosets_ms": round(lde_ms, 3),

@@ -347,10 +347,12 @@ hipError_t ntt_autotune(void *scratch, unsigned log_min, unsigned log_max, hipSt
   BC_TRY(ntt_prepare(std::min(log_max, 24u), s, nullptr));
   tuning_workspace ws(s);
   fe *buf = static_cast<fe *>(scratch);
-  for (unsigned L = std::max(log_min, 13u); L <= std::min(log_max, 24u); L++) {
+  for (unsigned L = std::max(log_min, NTT_LOG_TILE + 1); L <= std::min(log_max, 24u); L++) {
     unsigned best_first = 0;
     float best = 1e30f;
-    for (unsigned first = L - 12; first <= 12; first++) {
+    for (unsigned first = 1; first <= NTT_LOG_TILE; first++) {
+      if (!ntt_split_valid(L, first))
+        continue;
       ntt_set_first_pass(L, first);
       float ms = 0;
       auto record = [&](arena &ar) {
@@ -502,7 +504,7 @@ bc_error bc_amd_ntt_get_split(unsigned log_values_count, unsigned *first_pass_st
 
 bc_error bc_amd_ntt_set_split(unsigned log_values_count, unsigned first_pass_stages) {
   const unsigned L = log_values_count;
-  if (L > 28 || (first_pass_stages && (L < 13 || L > 24 || first_pass_stages > 12 || L - first_pass_stages > 12)))
+  if (L > 28 || (first_pass_stages && !ntt_split_valid(L, first_pass_stages)))
     return bc_error_invalid_value;
   ntt_set_first_pass(L, first_pass_stages);
   return bc_success;

@@ -169,6 +169,26 @@ template <class F> BC_HD fe sub2(const fe &a, const fe &b) {
 
 template <class F> BC_HD fe dbl2(const fe &a) { return add2<F>(a, a); }
 
+// Wider laziness for the NTT's butterflies, valid where 4m < 2^256 (Fr):
+// a + b for a, b < 2m (result < 4m, no reduction), and a - b + 2m for
+// a, b < 2m (result in (0, 4m), no select).  A Montgomery product of a value
+// < 4m and a canonical twiddle is < 2m again (4m^2/R + m < 1.76m for Fr).
+BC_HD fe add_raw(const fe &a, const fe &b) {
+  fe s;
+  add_limbs(s, a, b);
+  return s;
+}
+
+template <class F> BC_HD fe sub_raw2(const fe &a, const fe &b) {
+  fe t;
+  add_const(t, a, F::m2);
+  sub_limbs(t, t, b);
+  return t;
+}
+
+// a < 4m -> canonical
+template <class F> BC_HD fe reduce4(const fe &a) { return reduce<F>(reduce2<F>(a)); }
+
 // 2m - a for a < 2m (a lazy negation; 0 maps to 2m which is congruent to 0)
 template <class F> BC_HD fe neg2(const fe &a) {
   fe r;

@@ -36,8 +36,7 @@ namespace bc {
 
 namespace {
 
-constexpr unsigned NTT_LOG_TILE = 12; // 4096 elements per tile
-constexpr unsigned NTT_BLOCK = 1024;
+constexpr unsigned NTT_BLOCK = (1u << NTT_LOG_TILE) / 4; // one radix-4 group per thread and step
 constexpr unsigned TWIST_FINE_BITS = 12;
 
 struct twiddle_state {
@@ -161,13 +160,15 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
       const unsigned b = ((hi0 + g) << (s - s_lo)) | (t0 >> (s_hi - s));
       const fe E = tile[i0], O = tile[i1];
       if (!GS) {
-        const fe wo = s != 0 ? mul_lazy<F>(O, load_fe(tw + (1u << s) + b)) : O;
-        tile[i0] = add2<F>(E, wo);
-        tile[i1] = sub2<F>(E, wo);
+        // CT tile values are < 4m; each output is < 4m again
+        const fe wo = s != 0 ? mul_lazy<F>(O, load_fe(tw + (1u << s) + b)) : reduce2<F>(O);
+        const fe e = reduce2<F>(E);
+        tile[i0] = add_raw(e, wo);
+        tile[i1] = sub_raw2<F>(e, wo);
       } else {
-        const fe d = sub2<F>(E, O);
+        // GS tile values stay < 2m
         tile[i0] = add2<F>(E, O);
-        tile[i1] = s != 0 ? mul_lazy<F>(d, load_fe(tw + (1u << s) + b)) : d;
+        tile[i1] = s != 0 ? mul_lazy<F>(sub_raw2<F>(E, O), load_fe(tw + (1u << s) + b)) : sub2<F>(E, O);
       }
     }
     __syncthreads();
@@ -189,27 +190,37 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
       const fe *tw_s1 = tw + (2u << s) + 2 * b; // stage s+1 twiddles of blocks 2b, 2b+1
       fe x0 = tile[i0], x1 = tile[i1], x2 = tile[i2], x3 = tile[i3];
       if (!GS) {
+        // values < 4m in and out: the odd inputs of a butterfly go through a
+        // twiddle product (< 2m), the even ones through one reduction (< 2m)
         if (s != 0) {
           const fe wa = load_fe(tw + (1u << s) + b);
           x2 = mul_lazy<F>(x2, wa);
           x3 = mul_lazy<F>(x3, wa);
+        } else {
+          x2 = reduce2<F>(x2);
+          x3 = reduce2<F>(x3);
         }
-        fe y0 = add2<F>(x0, x2), y2 = sub2<F>(x0, x2);
-        fe y1 = add2<F>(x1, x3), y3 = sub2<F>(x1, x3);
-        y1 = mul_lazy<F>(y1, load_fe(tw_s1));
-        y3 = mul_lazy<F>(y3, load_fe(tw_s1 + 1));
-        tile[i0] = add2<F>(y0, y1);
-        tile[i1] = sub2<F>(y0, y1);
-        tile[i2] = add2<F>(y2, y3);
-        tile[i3] = sub2<F>(y2, y3);
+        x0 = reduce2<F>(x0);
+        x1 = reduce2<F>(x1);
+        const fe y0 = reduce2<F>(add_raw(x0, x2)), y2 = reduce2<F>(sub_raw2<F>(x0, x2));
+        const fe y1 = mul_lazy<F>(add_raw(x1, x3), load_fe(tw_s1));
+        const fe y3 = mul_lazy<F>(sub_raw2<F>(x1, x3), load_fe(tw_s1 + 1));
+        tile[i0] = add_raw(y0, y1);
+        tile[i1] = sub_raw2<F>(y0, y1);
+        tile[i2] = add_raw(y2, y3);
+        tile[i3] = sub_raw2<F>(y2, y3);
       } else {
-        const fe y0 = add2<F>(x0, x1), y1 = mul_lazy<F>(sub2<F>(x0, x1), load_fe(tw_s1));
-        const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub2<F>(x2, x3), load_fe(tw_s1 + 1));
-        fe z2 = sub2<F>(y0, y2), z3 = sub2<F>(y1, y3);
+        // values < 2m in and out; differences that feed a product skip the reduction
+        const fe y0 = add2<F>(x0, x1), y1 = mul_lazy<F>(sub_raw2<F>(x0, x1), load_fe(tw_s1));
+        const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub_raw2<F>(x2, x3), load_fe(tw_s1 + 1));
+        fe z2, z3;
         if (s != 0) {
           const fe wa = load_fe(tw + (1u << s) + b);
-          z2 = mul_lazy<F>(z2, wa);
-          z3 = mul_lazy<F>(z3, wa);
+          z2 = mul_lazy<F>(sub_raw2<F>(y0, y2), wa);
+          z3 = mul_lazy<F>(sub_raw2<F>(y1, y3), wa);
+        } else {
+          z2 = sub2<F>(y0, y2);
+          z3 = sub2<F>(y1, y3);
         }
         tile[i0] = add2<F>(y0, y2);
         tile[i1] = add2<F>(y1, y3);
@@ -228,7 +239,7 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
     fe v = tile[e];
     switch (post) {
     case POST_REDUCE:
-      v = reduce<F>(v);
+      v = reduce4<F>(v); // CT tiles hold values < 4m
       break;
     case POST_SCALE:
       v = mul<F>(v, scale);
@@ -258,19 +269,21 @@ struct pass_plan {
   unsigned s_lo, s_hi;
 };
 
-// Stage runs of the HBM passes: ceil(L/12) passes, balanced unless the tuning
-// table names the first pass of a two-pass size.
+// Stage runs of the HBM passes: ceil(L / NTT_LOG_TILE) passes, balanced;
+// the tuning table may name the first pass's stage count, and the rest are
+// then balanced over the remaining passes.
 std::vector<pass_plan> plan_passes(unsigned L) {
   std::vector<pass_plan> p;
-  const unsigned passes = (L + NTT_LOG_TILE - 1) / NTT_LOG_TILE;
-  if (passes == 2) {
-    const unsigned first = ntt_first_pass(L);
-    if (first && first < L && first <= NTT_LOG_TILE && L - first <= NTT_LOG_TILE)
-      return {{0, first}, {first, L}};
+  unsigned passes = ntt_passes(L), s = 0;
+  const unsigned first = passes >= 2 ? ntt_first_pass(L) : 0;
+  if (first && ntt_split_valid(L, first)) {
+    p.push_back({0, first});
+    s = first;
+    passes--;
   }
-  unsigned s = 0;
+  const unsigned rest = L - s;
   for (unsigned i = 0; i < passes; i++) {
-    const unsigned len = L / passes + (i < L % passes ? 1 : 0);
+    const unsigned len = rest / passes + (i < rest % passes ? 1 : 0);
     p.push_back({s, s + len});
     s += len;
   }

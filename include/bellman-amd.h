@@ -74,8 +74,10 @@ bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_m
 
 /* ------------------------------------------------------------------------ */
 /* Tuning (process-wide).  MSM: window width and piece size per log size;    */
-/* NTT: stages of the first of the two HBM passes for log sizes 13..24       */
-/* (0 = balanced).  Tuning changes no folded MSM point and no NTT output.    */
+/* NTT: stages of the first HBM pass for sizes of two or more passes (log    */
+/* sizes 11..28, at most 10 stages per pass; the other passes share the rest */
+/* evenly; 0 = balanced).  Tuning changes no folded MSM point and no NTT      */
+/* output.                                                                    */
 
 typedef struct {
   unsigned window_bits;

@@ -137,18 +137,19 @@ def test_ntt_batch_matches_plain_and_oracle(bellman, oracle):
 
 def test_ntt_stage_split_leaves_outputs_unchanged(bellman, oracle):
     _, dev = bellman
-    L = 16
-    x = oracle.random_fr(0xBC000046, 1 << L)
-    exp = oracle.ntt(x, L)
-    first = ctypes.c_uint()
-    B.call("bc_amd_ntt_get_split", ctypes.c_uint(L), ctypes.byref(first))
-    try:
-        for split in (4, 7, 12):
-            B.call("bc_amd_ntt_set_split", ctypes.c_uint(L), ctypes.c_uint(split))
-            assert np.array_equal(ntt(dev, x, L), exp), split
-            assert np.array_equal(ntt(dev, exp, L, bitrev=True, inverse=True), x), split
-    finally:
-        B.call("bc_amd_ntt_set_split", ctypes.c_uint(L), first)
+    # two passes (16 = 6..10 + rest) and three passes (21 = 1..10 + rest in two)
+    for L, splits in ((16, (6, 8, 10)), (21, (1, 5, 10))):
+        x = oracle.random_fr(0xBC000046 + L, 1 << L)
+        exp = oracle.ntt(x, L)
+        first = ctypes.c_uint()
+        B.call("bc_amd_ntt_get_split", ctypes.c_uint(L), ctypes.byref(first))
+        try:
+            for split in splits:
+                B.call("bc_amd_ntt_set_split", ctypes.c_uint(L), ctypes.c_uint(split))
+                assert np.array_equal(ntt(dev, x, L), exp), (L, split)
+                assert np.array_equal(ntt(dev, exp, L, bitrev=True, inverse=True), x), (L, split)
+        finally:
+            B.call("bc_amd_ntt_set_split", ctypes.c_uint(L), first)
 
 
 def test_plan_replays_match_eager_batches(bellman, oracle, shared_bases):
@@ -344,6 +345,39 @@ def test_server_tight_budget_runs_everything(bellman, oracle, shared_bases):
         srv.close()
 
 
+def test_server_shares_workspace_between_batches(bellman, oracle, shared_bases):
+    """A budget of about two jobs' footprints and three slots: batches take
+    workspace regions side by side, wait for one another when the workspace
+    is full, and every output equals the job's output when run alone."""
+    _, dev = bellman
+    db, _, logs = shared_bases
+    jobs = _server_jobs(dev, oracle, db) + _server_jobs(dev, oracle, db)
+    fps = []
+    for job, _ in jobs:
+        fp = ctypes.c_size_t()
+        B.call("bc_amd_job_footprint", ctypes.byref(job), ctypes.byref(fp))
+        fps.append(fp.value)
+    srv = B.Server(hbm_budget_bytes=2 * max(fps), max_in_flight=3)
+    try:
+        alone = []
+        for job, _ in jobs:
+            _clear(job)
+            assert srv.wait(srv.submit(job)).status == 0
+            alone.append(_output(dev, job))
+        for rnd in range(3):
+            for job, _ in jobs:
+                _clear(job)
+            tickets = [srv.submit(job) for job, _ in jobs]
+            stats = [srv.wait(t) for t in tickets]
+            assert all(s.status == 0 for s in stats), rnd
+            for s in stats:
+                assert s.submit_ns <= s.start_ns <= s.enqueued_ns <= s.done_ns and s.slot < 3
+            for (job, exp), a in zip(jobs, alone):
+                assert np.array_equal(_output(dev, job), a), (rnd, exp[0])
+    finally:
+        srv.close()
+
+
 def test_server_rejects_malformed_jobs(bellman):
     srv = B.Server(hbm_budget_bytes=1 << 20)
     try:
@@ -382,7 +416,7 @@ def test_autotune_sets_valid_tuning_and_keeps_results(bellman, oracle, shared_ba
         for L in (13, 14):
             f = ctypes.c_uint()
             B.call("bc_amd_ntt_get_split", ctypes.c_uint(L), ctypes.byref(f))
-            assert L - 12 <= f.value <= 12
+            assert 1 <= f.value <= 10 and L - f.value <= 10  # two passes at these sizes
             x = oracle.random_fr(0xBC00004E + L, 1 << L)
             assert np.array_equal(ntt(dev, x, L), oracle.ntt(x, L))
     finally:

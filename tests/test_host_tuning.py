@@ -40,11 +40,14 @@ def test_tuning_validation(B):
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(2, 32)) == B.bc_error_invalid_value
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(25, 32)) == B.bc_error_invalid_value
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(10, 0)) == B.bc_error_invalid_value
-    # two-pass sizes only, both passes at most 12 stages
-    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(12), ctypes.c_uint(6)) == B.bc_error_invalid_value
+    # sizes of two or more passes only; every pass at most 10 stages
+    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(10), ctypes.c_uint(5)) == B.bc_error_invalid_value  # one pass
     assert lib.bc_amd_ntt_set_split(ctypes.c_uint(24), ctypes.c_uint(11)) == B.bc_error_invalid_value
-    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(20), ctypes.c_uint(13)) == B.bc_error_invalid_value
+    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(24), ctypes.c_uint(3)) == B.bc_error_invalid_value  # 21 left for 2 passes
+    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(20), ctypes.c_uint(9)) == B.bc_error_invalid_value
     assert lib.bc_amd_ntt_set_split(ctypes.c_uint(20), ctypes.c_uint(0)) == B.bc_success
+    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(24), ctypes.c_uint(4)) == B.bc_success
+    assert lib.bc_amd_ntt_set_split(ctypes.c_uint(24), ctypes.c_uint(0)) == B.bc_success
     assert lib.bc_amd_tuning_load(b"/nonexistent/tuning.txt") == B.bc_error_invalid_value
 
 
