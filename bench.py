@@ -284,6 +284,28 @@ def headline_msm(lab, L, steps, warmup):
     return ms, ok, tuned_ms, tuned_ok
 
 
+# ---------------------------------------------------------------- PN (the served mix's 4 x 2^20 job)
+def headline_pn(lab, steps, warmup, cols=4, log_rows=20):
+    """Permutation polynomials for 4 x 2^20 cells (random cycles 1..64) through
+    the plain C ABI; the roofline counts 36 B per cell (a 4 B index read and a
+    32 B value written)."""
+    dev = lab.dev
+    cells = cols << log_rows
+    d_idx = dev.upload(pn_indexes(0xBC00000F, cols, log_rows))
+    d_ks = lab.upload_mont(uniform_fr(0xBC000010, cols))
+    d_out = dev.malloc(32 * cells)
+    for _ in range(warmup):
+        pn_call(lab, d_idx, d_ks, d_out, cols, log_rows)
+    dev.sync()
+    ms = lab.time_ms(lambda: pn_call(lab, d_idx, d_ks, d_out, cols, log_rows), steps)
+    for p in (d_idx, d_ks, d_out):
+        dev.free(p)
+    achieved = 36.0 * cells / (ms * 1e-3) / 1e9
+    return {"workload": "PN %d x 2^%d (random cycles 1..64)" % (cols, log_rows), "ms": round(ms, 3),
+            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes": 36 * cells, "kernel": "whole pn_generate_permutation_polynomials call"}}
+
+
 # ---------------------------------------------------------------- batched (64 MSM + 32 NTT)
 def batched(lab, reps):
     dev = lab.dev
@@ -609,7 +631,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--log-msm", type=int, default=MSM_METRIC_LOG)
     ap.add_argument("--log-ntt", type=int, default=24)
-    ap.add_argument("--parts", default="tune,ntt,msm,batched,served,cpu",
+    ap.add_argument("--parts", default="tune,ntt,msm,pn,batched,served,cpu",
                     help="comma list of: tune, ntt, msm, batched, served, cpu, copy (a 2^24-element copy kernel, for counter scaling)")
     ap.add_argument("--served-jobs", type=int, default=120)
     ap.add_argument("--served-budget-gib", type=float, default=160.0)
@@ -667,6 +689,9 @@ def main():
             out["roofline_msm"] = {"bound": "valu (v_mad_u64_u32 issue)", "kernel": "whole msm_execute_async call", "achieved": round(tps, 2),
                                    "peak": VALU_MAD_PEAK_T, "unit": "T 32x32-bit products/s", "frac": round(tps / VALU_MAD_PEAK_T, 4)}
         out.setdefault("parity", {}).update({"msm_closed_form": msm_ok, "msm_autotuned_closed_form": tuned_ok})
+    if "pn" in parts:
+        out["pn"] = headline_pn(lab, max(1, args.steps), args.warmup)
+        log("pn: %s" % out["pn"])
     if "ntt" in parts and "msm" in parts:
         step = out["msm_ms"] + out["ntt_ms"]
         out.update({"value": round(step, 3), "unit": "ms per step (one 2^24 G1 MSM + one 2^24 Fr NTT)", "ms_per_step": round(step, 3)})

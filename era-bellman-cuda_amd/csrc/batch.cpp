@@ -191,11 +191,12 @@ size_t ntt_footprint(const ntt_job &j) {
   return j.log_ext ? (size_t)(32 * (4096 + coarse) + 4096) : 4096;
 }
 
-// pn_generate_device: keys, cells, sorted copies, flags, scan, run tables
-// (11 x 4 B per cell), the sort's ping-pong buffers (8 B) and histograms.
+// pn_generate_device: the (variable, cell) pairs and their sorted copy (16 B
+// per cell), the sort's ping-pong buffer (8 B) and histograms (<= 2 B), then
+// flags, scan and run ends (12 B) beside the sorted copy.
 size_t pn_footprint(unsigned cols, unsigned log_rows) {
   const uint64_t n = (uint64_t)cols << log_rows;
-  return (size_t)(53 * n + (1u << 20));
+  return (size_t)(46 * n + (1u << 20));
 }
 
 hipError_t measure_scratch(hipStream_t s, const std::function<hipError_t(arena &)> &record, size_t *bytes) {

@@ -20,47 +20,43 @@ namespace {
 
 constexpr unsigned PN_BLOCK = 256;
 
-// keys[row*cols + col] = indexes[col*rows + row]; cells[...] = col*rows + row
-__global__ void pn_transpose_kernel(const uint32_t *indexes, uint32_t *keys, uint32_t *cells, unsigned cols, unsigned log_rows) {
+// entries[row*cols + col] = (cell << 32) | indexes[cell], cell = col*rows + row:
+// the variable index is the sort key, the cell rides along in the high word
+__global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, unsigned cols, unsigned log_rows) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned cells_count = cols << log_rows;
   if (p >= cells_count)
     return;
   const unsigned row = p / cols, col = p % cols;
   const unsigned cell = (col << log_rows) + row;
-  keys[p] = indexes[cell];
-  cells[p] = cell;
+  entries[p] = (uint64_t)cell << 32 | indexes[cell];
 }
 
-__global__ void pn_run_flags_kernel(const uint32_t *sorted_keys, uint32_t *flags, unsigned n) {
+__global__ void pn_run_flags_kernel(const uint64_t *sorted, uint32_t *flags, unsigned n) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < n)
-    flags[j] = (j == 0 || sorted_keys[j] != sorted_keys[j - 1]) ? 1u : 0u;
+    flags[j] = (j == 0 || (uint32_t)sorted[j] != (uint32_t)sorted[j - 1]) ? 1u : 0u;
 }
 
-// run_of[j] = index of the run holding sorted position j; start/end of each run
-__global__ void pn_runs_kernel(const uint32_t *flags, const uint32_t *excl, uint32_t *run_of, uint32_t *run_start, uint32_t *run_end, unsigned n) {
+// run r (r = excl[j] + flags[j] - 1 at sorted position j) ends at run_end[r]
+__global__ void pn_run_ends_kernel(const uint32_t *flags, const uint32_t *excl, uint32_t *run_end, unsigned n) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n)
-    return;
-  const uint32_t r = excl[j] + flags[j] - 1;
-  run_of[j] = r;
-  if (flags[j])
-    run_start[r] = j;
-  if (j == n - 1 || flags[j + 1])
-    run_end[r] = j + 1;
+  if (j < n && (j == n - 1 || flags[j + 1]))
+    run_end[excl[j] + flags[j] - 1] = j + 1;
 }
 
-__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint32_t *sorted_cells, const uint32_t *run_of, const uint32_t *run_start,
+// Sorted position j of a run of length len at offset o sends its value to
+// position o + ((j - o + len - 1) mod len): the previous position, or the
+// run's last one for its first position; run 0 maps to itself
+// (src/pn_kernels.cu:89-121).
+__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint64_t *sorted, const uint32_t *flags, const uint32_t *excl,
                                    const uint32_t *run_end, fe *target, unsigned n, unsigned log_rows) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n)
     return;
-  const uint32_t r = run_of[j], o = run_start[r], len = run_end[r] - o;
-  const uint32_t src_in = j - o;
-  const uint32_t dst_in = r == 0 ? src_in : (src_in + len - 1) % len;
-  const uint32_t src_cell = sorted_cells[j];
-  const uint32_t dst_cell = sorted_cells[o + dst_in];
+  const uint32_t dst = j < run_end[0] ? j : (flags[j] ? run_end[excl[j]] - 1 : j - 1);
+  const uint32_t src_cell = (uint32_t)(sorted[j] >> 32);
+  const uint32_t dst_cell = (uint32_t)(sorted[dst] >> 32);
   const uint32_t row = src_cell & ((1u << log_rows) - 1), col = src_cell >> log_rows;
   const fe tw = table_pow(w, (row << (POW_LOG_RANGE - log_rows)) & ((1u << POW_LOG_RANGE) - 1));
   store_fe(target + dst_cell, mul<FrParams>(tw, load_fe(scalars + col)));
@@ -78,31 +74,28 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   hipStream_t s = ar.stream();
   const unsigned n = cols << L;
   const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
-  uint32_t *keys = nullptr, *cells = nullptr, *skeys = nullptr, *scells = nullptr, *flags = nullptr, *excl = nullptr;
-  uint32_t *run_of = nullptr, *run_start = nullptr, *run_end = nullptr;
-  BC_TRY(ar.get(&keys, n));
-  BC_TRY(ar.get(&cells, n));
-  BC_TRY(ar.get(&skeys, n));
-  BC_TRY(ar.get(&scells, n));
-  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, keys, cells, cols, L);
+  uint64_t *sorted = nullptr, *entries = nullptr;
+  uint32_t *flags = nullptr, *excl = nullptr, *run_end = nullptr;
+  BC_TRY(ar.get(&sorted, n));
+  BC_TRY(ar.get(&entries, n)); // on top of the stack: reclaimed after the sort
+  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, entries, cols, L);
   BC_LAUNCH_CHECK();
-  BC_TRY(radix_sort_pairs(keys, skeys, cells, scells, n, bits, ar));
-  BC_TRY(ar.put(keys));
-  BC_TRY(ar.put(cells));
+  const seg_group all{0, n, 1};
+  BC_TRY(segmented_sort_kv(entries, sorted, &all, 1, bits, ar));
+  BC_TRY(ar.put(entries));
   BC_TRY(ar.get(&flags, n));
   BC_TRY(ar.get(&excl, n));
-  pn_run_flags_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(skeys, flags, n);
+  pn_run_flags_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, flags, n);
   BC_LAUNCH_CHECK();
   BC_TRY(scan_u32_exclusive(flags, excl, n, ar));
-  BC_TRY(ar.get(&run_of, n));
-  BC_TRY(ar.get(&run_start, n));
   BC_TRY(ar.get(&run_end, n));
-  pn_runs_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(flags, excl, run_of, run_start, run_end, n);
+  pn_run_ends_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(flags, excl, run_end, n);
   BC_LAUNCH_CHECK();
-  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, scells, run_of, run_start, run_end, target, n, L);
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, sorted, flags, excl, run_end, target, n, L);
   BC_LAUNCH_CHECK();
-  for (uint32_t *p : {skeys, scells, flags, excl, run_of, run_start, run_end})
+  for (uint32_t *p : {run_end, excl, flags})
     BC_TRY(ar.put(p));
+  BC_TRY(ar.put(sorted));
   return hipSuccess;
 }
 

@@ -1,6 +1,6 @@
-// Stable LSD radix sort of u32 keys (optionally with u32 values), replacing
-// the reference's cub DeviceRadixSort calls (src/bellman-cuda-cub.cu:9-23).
-// Sorts on key bits [0, key_bits); equal keys keep their input order.
+// Stable LSD radix sorts replacing the reference's cub DeviceRadixSort calls
+// (src/bellman-cuda-cub.cu:9-23): u32 keys, or u64 key-value entries.  Sorts
+// on key bits [0, key_bits); equal keys keep their input order.
 #pragma once
 
 #include "common.h"
@@ -8,9 +8,6 @@
 namespace bc {
 
 hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n, unsigned key_bits, arena &ar);
-
-hipError_t radix_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, size_t n, unsigned key_bits,
-                            arena &ar);
 
 } // namespace bc
 

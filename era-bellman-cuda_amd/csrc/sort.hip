@@ -339,12 +339,4 @@ hipError_t radix_sort_keys(const uint32_t *keys_in, uint32_t *keys_out, size_t n
   return n ? segmented_sort_pairs(keys_in, keys_out, nullptr, nullptr, &all, 1, key_bits, ar) : hipSuccess;
 }
 
-hipError_t radix_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, size_t n, unsigned key_bits,
-                            arena &ar) {
-  if (n >= (1ull << 32))
-    return hipErrorInvalidValue;
-  const seg_group all{0, (uint32_t)n, 1};
-  return n ? segmented_sort_pairs(keys_in, keys_out, values_in, values_out, &all, 1, key_bits, ar) : hipSuccess;
-}
-
 } // namespace bc
