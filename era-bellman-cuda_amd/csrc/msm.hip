@@ -452,9 +452,21 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
   const uint32_t id = id_all - sg.thread_begin;
   const unsigned s = sg.s, t = (s + 1) >> 1, hs = s - t;
   const uint32_t per_part = (2 * sg.X) << t;
-  const uint32_t part = id / per_part, rem = id % per_part;
-  const uint32_t xo = rem >> t, j = rem & ((1u << t) - 1);
-  const uint32_t x = xo >> 1, half = xo & 1;
+  uint32_t part = id / per_part;
+  const uint32_t rem = id % per_part;
+  uint32_t j = rem & ((1u << t) - 1), x = rem >> (t + 1);
+  const uint32_t half = (rem >> t) & 1;
+  if (half) {
+    // Half-1 sums run along a row (consecutive u); the threads of this half
+    // are renumbered so that neighbouring lanes take neighbouring parts of
+    // one row and their loads are contiguous: (part, x, j) -> L -> (x, j, part)
+    // with part fastest.  D is a power of two.
+    const uint32_t log_d = __builtin_ctz(sg.D);
+    const uint32_t L = ((part * sg.X + x) << t) + j;
+    part = L & (sg.D - 1);
+    j = (L >> log_d) & ((1u << t) - 1);
+    x = L >> (log_d + t);
+  }
   const g1_xyzz *src = sg.A + ((size_t)x << s);
   g1_xyzz acc = xyzz_infinity();
   if (half == 0) {
@@ -465,7 +477,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
     for (uint32_t u = part; u < (1u << t); u += sg.D)
       acc = xyzz_add(acc, load_xyzz(src + ((j << t) | u)));
   }
-  store_xyzz(sg.out + id, acc);
+  store_xyzz(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, acc);
 }
 
 // out[i] += out[i + half] for i < half, per segment
