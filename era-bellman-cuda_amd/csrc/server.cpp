@@ -56,7 +56,7 @@ struct batch {
   std::vector<request *> reqs;
   size_t bytes = 0;         // summed footprints (admission)
   size_t region = 0;        // offset of the workspace region
-  size_t region_bytes = 0;  // exact size; 0 when the batch ran from the pool
+  size_t region_bytes = 0;  // exact size; 0 when it took no region (no temporaries, or memory from the pool)
 };
 
 // First-fit allocator of workspace regions (byte offsets, 256-byte granules).
@@ -297,7 +297,9 @@ private:
     hipStream_t s = streams_[b.slot];
     size_t need = 0;
     BC_TRY(size_of(w, s, &need));
-    const bool placed = heap_.take(need, &b.region);
+    // a batch with no temporaries (plain NTTs) takes no region
+    b.region = 0;
+    const bool placed = need == 0 || heap_.take(need, &b.region);
     if (!placed && !inflight_.empty()) {
       *deferred = true;
       return hipSuccess;
@@ -308,7 +310,7 @@ private:
       r->stats.batch_jobs = (unsigned)b.reqs.size();
       r->stats.slot = b.slot;
     }
-    b.region_bytes = placed ? need : 0;
+    b.region_bytes = placed ? need : 0; // returned to the heap when the batch completes
     arena ar = placed ? arena::bump_mode(static_cast<char *>(workspace_) + b.region, need, s) : arena::pool_mode(pool_, s);
     BC_TRY(record(w, ar));
     BC_TRY(hipEventRecord(events_[b.slot], s));

@@ -91,3 +91,44 @@ def test_msm_config_2_20(bellman, oracle):
     R = oracle.R
     t = (s0 * sum(ks) + d * sum(i * kv for i, kv in enumerate(ks))) % R
     assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], oracle.mul_generator(t))
+
+
+def test_msms_overlap_on_four_streams(bellman, oracle):
+    """Four plain msm_execute_async calls in flight on four streams at once (as
+    tests/bc_test.cu:126-170 overlaps them); each result equals the same MSM
+    run alone, byte for byte, and folds to its closed form."""
+    import ctypes
+
+    B, dev = bellman
+    s0, d = 0xBC000061, 0x2F3E4D5C6B7A8999
+    bases = oracle.gen_bases_progression(s0, d, 1 << 14)
+    db = dev.upload(bases)
+    logs = [(s0 + i * d) % oracle.R for i in range(1 << 14)]
+    sizes = [14, 12, 13, 10]
+    scal = [oracle.msm_scalars(0xBC000062 + j, 1 << L) for j, L in enumerate(sizes)]
+    alone = [msm(dev, db, k, L) for k, L in zip(scal, sizes)]
+    streams, d_s, d_r = [], [], []
+    try:
+        for j, L in enumerate(sizes):
+            st = B.bc_stream()
+            B.call("bc_stream_create", ctypes.byref(st), ctypes.c_bool(False))
+            streams.append(st)
+            d_s.append(dev.upload(scal[j]))
+            d_r.append(dev.malloc(254 * 96))
+        for j, L in enumerate(sizes):
+            cfg = B.msm_configuration()
+            cfg.mem_pool, cfg.stream = dev.pool, streams[j]
+            cfg.bases, cfg.scalars, cfg.results = db, d_s[j], d_r[j]
+            cfg.log_scalars_count = L
+            B.call("msm_execute_async", cfg)
+        for st in streams:
+            B.call("bc_stream_synchronize", st)
+        for j, L in enumerate(sizes):
+            got = dev.download(d_r[j], (254, 24))
+            assert np.array_equal(got, alone[j]), L
+            assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, logs[: 1 << L], scal[j]))
+    finally:
+        for st in streams:
+            B.call("bc_stream_destroy", st)
+        for p in d_s + d_r + [db]:
+            dev.free(p)

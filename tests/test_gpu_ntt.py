@@ -76,3 +76,19 @@ def test_ntt_2_24_round_trip_and_sampled_evaluations(bellman, oracle):
         assert np.array_equal(y[brev(k, L)], oracle.poly_eval(x, z)[0])
     z = ntt(dev, y, L, bitrev=True, inverse=True)
     assert np.array_equal(z, x)
+
+
+def test_lde_assembled_in_whole_domain_bit_reversed_order(bellman, oracle):
+    """Prover-side assembly of a 4x LDE at 2^16: running coset brev_2(c) for
+    output block c gives, at position j of the 2^18 outputs, P(g w^brev(j)),
+    i.e. the size-2^18 natural -> bit-reversed transform of (g^i x_i) padded
+    with zeros (the ordering the reference's lde_4 answers follow,
+    tests/ntt_test.cu:633-725)."""
+    B, dev = bellman
+    L, e = 16, 2
+    n = 1 << L
+    x = oracle.random_fr(0xBC000060, n)
+    got = np.concatenate([ntt(dev, x, L, log_ext=e, coset=brev(c, e)) for c in range(1 << e)])
+    y = np.zeros(((1 << e) * n, 8), dtype=np.uint32)
+    y[:n] = oracle.multiply_by_powers(x, oracle.to_mont([7]))
+    assert np.array_equal(got, oracle.ntt(y, L + e))
