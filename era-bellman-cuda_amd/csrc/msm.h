@@ -29,4 +29,8 @@ struct msm_job {
 // temporaries come from `ar`, everything is ordered on ar.stream().
 hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar);
 
+// Threads (one XYZZ point of scratch each) over all bucket-split levels of one
+// job with window width c.
+uint64_t msm_split_threads(unsigned c);
+
 } // namespace bc

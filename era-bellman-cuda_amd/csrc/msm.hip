@@ -78,7 +78,7 @@ namespace {
 constexpr uint32_t SIGN = 0x80000000u;
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
-constexpr unsigned SPLIT_TERMS = 16;
+constexpr unsigned SPLIT_TERMS = 16; // most terms one split thread adds
 constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many pieces are summed by one thread
 
 struct job_desc {
@@ -549,6 +549,34 @@ __global__ void result_kernel(const result_table jobs) {
 
 } // namespace
 
+// Parts D_lo of the low half of a split level (X windows of `width` bits):
+// each split thread adds 2^hs / D_lo terms, SPLIT_TERMS where the level has
+// work enough for 2^16 threads and fewer (down to 2) on small levels, whose
+// serial chains would otherwise set the time.  A function of the job alone,
+// so a job's sums are the same alone and in any batch.
+namespace {
+
+uint32_t split_parts(uint32_t X, unsigned width) {
+  const unsigned hs = width - (width + 1) / 2;
+  const uint64_t level_terms = (uint64_t)(2 * X) << width;
+  unsigned terms = SPLIT_TERMS;
+  while (terms > 2 && level_terms / terms < (1u << 16))
+    terms >>= 1;
+  return (1u << hs) > terms ? (1u << hs) / terms : 1u;
+}
+
+} // namespace
+
+uint64_t msm_split_threads(unsigned c) {
+  uint64_t total = 0;
+  uint32_t X = (254 - 1) / c + 1;
+  for (unsigned width = c - 1; width > 1; width = (width + 1) / 2, X *= 2) {
+    const unsigned t = (width + 1) / 2, hs = width - t;
+    total += ((uint64_t)split_parts(X, width) << (t - hs)) * ((uint64_t)(2 * X) << t);
+  }
+  return total;
+}
+
 hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   hipStream_t s = ar.stream();
   if (count == 0)
@@ -680,7 +708,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
       if (width[j] <= 1)
         continue;
       const unsigned t = (width[j] + 1) / 2, hs = width[j] - t;
-      const uint32_t D_lo = (1u << hs) > SPLIT_TERMS ? (1u << hs) / SPLIT_TERMS : 1u;
+      const uint32_t D_lo = split_parts(X[j], width[j]);
       split_seg &sg = st.seg[st.count++];
       sg.A = cur[j];
       sg.X = X[j];
