@@ -31,9 +31,9 @@ constexpr unsigned TUNE_MAX_LOG = 28;
 // than the plain call's pay off in batches: fewer buckets to reduce per job.
 msm_tuning msm_builtin_tuning(unsigned l) {
   static const msm_tuning measured[] = {
-      {5, 32},  /* 12 */ {6, 16},  /* 13 */ {6, 32},  /* 14 */ {9, 16},  /* 15 */ {11, 16}, /* 16 */
-      {11, 16}, /* 17 */ {11, 16}, /* 18 */ {12, 16}, /* 19 */ {13, 16}, /* 20 */ {14, 16}, /* 21 */
-      {15, 16}, /* 22 */ {15, 32}, /* 23 */ {17, 32}, /* 24 */
+      {5, 32},  /* 12 */ {5, 16},  /* 13 */ {5, 32},  /* 14 */ {7, 32},  /* 15 */ {9, 32},  /* 16 */
+      {9, 32},  /* 17 */ {11, 16}, /* 18 */ {11, 32}, /* 19 */ {13, 16}, /* 20 */ {14, 16}, /* 21 */
+      {15, 32}, /* 22 */ {15, 32}, /* 23 */ {17, 32}, /* 24 */
   };
   if (l < 12)
     return {std::max(3u, l > 7 ? l - 7 : 3u), 16};

@@ -27,7 +27,7 @@ def test_msm_tuning_builtin_table(B):
     pkg = os.path.dirname(os.path.dirname(B.__file__))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, env=dict(os.environ, PYTHONPATH=pkg))
     table = {int(a): (int(b), int(c)) for a, b, c in (line.split() for line in out.stdout.splitlines())}
-    assert table[12] == (5, 32) and table[16] == (11, 16) and table[20] == (13, 16) and table[24] == (17, 32)
+    assert table[12] == (5, 32) and table[16] == (9, 32) and table[20] == (13, 16) and table[24] == (17, 32)
     assert table[25] == (20, 32) and table[26] == (22, 32) and table[28] == (22, 32)
     for L, (c, piece) in table.items():
         assert 3 <= c <= 22 and piece in (16, 32)
