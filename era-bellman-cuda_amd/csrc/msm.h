@@ -8,9 +8,10 @@
 namespace bc {
 
 // Points per accumulating thread of the plain call at the reference window
-// widths: 16 up to 2^23 points, 64 above (measured on MI355X; shorter chains
-// pay off while a launch holds too few threads to hide latency).
-constexpr unsigned msm_plain_piece(unsigned log_n) { return log_n <= 23 ? 16u : 64u; }
+// widths, measured on MI355X: 8 for 2^15..2^17 points (the bucket of the
+// scalars 0 and 1 splits into more chains), 16 up to 2^23, 64 above (as fast
+// as 8 at 2^24, with an eighth of the partial sums in memory).
+constexpr unsigned msm_plain_piece(unsigned log_n) { return log_n >= 24 ? 64u : (log_n >= 15 && log_n <= 17 ? 8u : 16u); }
 constexpr unsigned MSM_MAX_JOBS = 64;
 
 // Window width table of the reference (src/msm.cu:12-37).
