@@ -89,7 +89,7 @@ BC_HD g1_xyzz xyzz_add_affine(const g1_xyzz &p, const g1_affine &q) {
   const fe PPP = mul_lazy<Fq>(P, PP);
   const fe Q = mul_lazy<Fq>(p.x, PP);
   const fe X3 = sub2<Fq>(sub2<Fq>(mul_lazy<Fq>(R, R), PPP), dbl2<Fq>(Q));
-  const fe Y3 = sub2<Fq>(mul_lazy<Fq>(R, sub2<Fq>(Q, X3)), mul_lazy<Fq>(p.y, PPP));
+  const fe Y3 = mul2_lazy<Fq>(R, sub2<Fq>(Q, X3), p.y, neg2<Fq>(PPP)); // R (Q - X3) - Y1 PPP
   return {X3, Y3, mul_lazy<Fq>(p.zz, PP), mul_lazy<Fq>(p.zzz, PPP)};
 }
 
@@ -111,7 +111,7 @@ BC_HD g1_xyzz xyzz_add(const g1_xyzz &p, const g1_xyzz &q) {
   const fe PPP = mul_lazy<Fq>(P, PP);
   const fe Q = mul_lazy<Fq>(U1, PP);
   const fe X3 = sub2<Fq>(sub2<Fq>(mul_lazy<Fq>(R, R), PPP), dbl2<Fq>(Q));
-  const fe Y3 = sub2<Fq>(mul_lazy<Fq>(R, sub2<Fq>(Q, X3)), mul_lazy<Fq>(S1, PPP));
+  const fe Y3 = mul2_lazy<Fq>(R, sub2<Fq>(Q, X3), S1, neg2<Fq>(PPP)); // R (Q - X3) - S1 PPP
   return {X3, Y3, mul_lazy<Fq>(p.zz, mul_lazy<Fq>(q.zz, PP)), mul_lazy<Fq>(p.zzz, mul_lazy<Fq>(q.zzz, PPP))};
 }
 

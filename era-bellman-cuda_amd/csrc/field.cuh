@@ -198,8 +198,8 @@ template <class F> BC_HD fe neg2(const fe &a) {
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
-// Montgomery product a*b/R by finely integrated product scanning, one asm
-// statement per column (generated; see gen_mont_asm.py).  Inputs < 2m, output < 2m.
+// Montgomery products a*b/R and (a*b + c*d)/R by finely integrated product
+// scanning, one asm statement per column (generated; see gen_mont_asm.py).
 #include "mont_asm.h"
 #endif
 
@@ -235,6 +235,19 @@ template <class F> BC_HD fe mul_lazy(const fe &a, const fe &b) {
   for (int i = 0; i < 8; i++)
     r.v[i] = t[i];
   return r;
+#endif
+}
+
+// (a*b + c*d)/R with one reduction for both products; inputs < 2m (or = 2m),
+// output < 2m.  The reduction's result is below (8m^2 + Rm)/R < 2.52m for
+// both fields, so one conditional subtraction of 2m restores the lazy bound.
+// Used where a formula subtracts a product from another (Y3 of the XYZZ
+// additions, as the reference's wide form does, src/ec.cuh:582-588).
+template <class F> BC_HD fe mul2_lazy(const fe &a, const fe &b, const fe &c, const fe &d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return reduce2<F>(mul2_lazy_device<F>(a, b, c, d));
+#else
+  return add2<F>(mul_lazy<F>(a, b), mul_lazy<F>(c, d));
 #endif
 }
 

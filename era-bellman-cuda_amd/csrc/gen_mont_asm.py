@@ -1,23 +1,30 @@
-"""Generates mont_asm.h: the 8 x 32-bit-limb Montgomery product used by
+"""Generates mont_asm.h: the 8 x 32-bit-limb Montgomery products used by
 field.cuh on gfx950, as a few inline-asm statements per column.
 
-Column k of the finely integrated product scan accumulates a_i * b_(k-i) and
-m_i * N_(k-i) into a 64-bit register pair with v_mad_u64_u32.  Every carry-out
-goes to an SGPR pair and is added into the column's overflow word c2 by a
-v_addc_co_u32_e64 that reads the pair.  gfx950 needs wait states between a VALU
-write of an SGPR and a VALU read of it as a carry (the compiler itself keeps two
-instructions between them), so the carries rotate over three SGPR pairs and
-each add is scheduled at least two instructions after the multiply that wrote
-its pair; the adds write their own (always zero) carry-out to a fourth pair
-that nothing reads.  Only where a column is too short to fill the gap does the
-scheduler emit s_nop.  The first product of a column cannot overflow (the
-carried-in value is below 2^37), so it needs no carry.
+mul_lazy_device(a, b) = a b / R and mul2_lazy_device(a, b, c, d) =
+(a b + c d) / R (one reduction for the sum of two products).
 
-Statements: per low column k < 8 one for a*b and m_0..m_(k-1) * N, then (after
-m_k = lo(acc) * inv in C++) one for m_k * N_0 that also drains the carries the
-first left pending; per high column one statement.  Between statements the
-compiler emits at least the v_mul_lo_u32 of m_k, so carries pending across a
-statement boundary count one instruction of distance.
+Column k of the finely integrated product scan accumulates a_i * b_(k-i) (and
+c_i * d_(k-i)) and m_i * N_(k-i) into a 64-bit register pair with
+v_mad_u64_u32.  Every carry-out goes to an SGPR pair and is added into the
+column's overflow word c2 by a v_addc_co_u32_e64 that reads the pair.  gfx950
+needs wait states between a VALU write of an SGPR and a VALU read of it as a
+carry (the compiler itself keeps two instructions between them), so the
+carries rotate over three SGPR pairs and each add is scheduled at least two
+instructions after the multiply that wrote its pair; the adds write their own
+(always zero) carry-out to a fourth pair that nothing reads.  Only where a
+column is too short to fill the gap does the scheduler emit s_nop.
+
+The first product of each column keeps no carry.  From column 1 on it is an
+m_i * N_j product: every modulus limb is below 0xF0000001, so it is below
+2^64 - 2^60, and the carried-in value is below 2^37 (a column holds at most 24
+products), so their sum stays below 2^64.  Column 0 starts from zero.
+
+Statements: per low column k < 8 one for the products and m_0..m_(k-1) * N,
+then (after m_k = lo(acc) * inv in C++) one for m_k * N_0 that also drains
+the carries the first left pending; per high column one statement.  Between
+statements the compiler emits at least the v_mul_lo_u32 of m_k, so carries
+pending across a statement boundary count one instruction of distance.
 
 Run: python3 gen_mont_asm.py > mont_asm.h
 """
@@ -74,12 +81,8 @@ class Statement:
         return [p for p, _ in self.pending]
 
 
-def operand_a(i):
-    return "a%d" % i, '[a%d] "v"(a.v[%d])' % (i, i)
-
-
-def operand_b(j):
-    return "b%d" % j, '[b%d] "v"(b.v[%d])' % (j, j)
+def operand(name, idx, src):
+    return "%s%d" % (name, idx), '[%s%d] "v"(%s.v[%d])' % (name, idx, src, idx)
 
 
 def operand_m(i):
@@ -108,13 +111,11 @@ def render(st, incoming, c2_was_started):
     return '  asm("%s"\n      : %s\n      : %s);' % (text, ", ".join(outs), ",\n        ".join(ins))
 
 
-def emit():
+def emit_function(name, streams):
+    """streams: the products' operand names, e.g. [("a", "b")] or [("a", "b"), ("u", "w")]."""
+    params = ", ".join("const fe &%s" % x for pair in streams for x in pair)
     out = []
-    out.append("// Generated by gen_mont_asm.py: the 8 x 32-bit-limb Montgomery product")
-    out.append("// (finely integrated product scanning) for gfx950.")
-    out.append("// Included by field.cuh inside namespace bc, device compilation only.")
-    out.append("")
-    out.append("template <class F> __device__ __forceinline__ fe mul_lazy_device(const fe &a, const fe &b) {")
+    out.append("template <class F> __device__ __forceinline__ fe %s(%s) {" % (name, params))
     out.append("  uint32_t m[8];")
     out.append("  fe r;")
     out.append("  uint64_t acc = 0;")
@@ -123,25 +124,35 @@ def emit():
     for k in range(15):
         low = k < 8
         out.append("  // column %d" % k)
-        if low:
-            ab = [(i, k - i) for i in range(0, k + 1)]
-            mp = [(i, k - i) for i in range(0, k)]
-        else:
-            ab = [(i, k - i) for i in range(k - 7, 8)]
-            mp = [(i, k - i) for i in range(k - 7, 8)]
+        rng = range(0, k + 1) if low else range(k - 7, 8)
+        prods = [(x, i, y, k - i) for (x, y) in streams for i in rng]
+        mp = [(i, k - i) for i in (range(0, k) if low else range(k - 7, 8))]
         st = Statement([], False)
-        for n, (i, j) in enumerate(ab):
-            xa, ta = operand_a(i)
-            xb, tb = operand_b(j)
+
+        def mad_prod(pr, carry):
+            x, i, y, j = pr
+            xa, ta = operand(x, i, x)
+            xb, tb = operand(y, j, y)
             st.use(xa, ta)
             st.use(xb, tb)
-            st.mad(xa, xb, carry=n > 0)
-        for i, j in mp:
+            st.mad(xa, xb, carry=carry)
+
+        def mad_mn(i, j, carry):
             xm, tm = operand_m(i)
             xn, tn = operand_n(j)
             st.use(xm, tm)
             st.use(xn, tn)
-            st.mad(xm, xn, carry=True)
+            st.mad(xm, xn, carry=carry)
+
+        if mp:
+            mad_mn(mp[0][0], mp[0][1], carry=False)
+            for pr in prods:
+                mad_prod(pr, carry=True)
+            for i, j in mp[1:]:
+                mad_mn(i, j, carry=True)
+        else:  # column 0: the accumulator starts at zero
+            for n, pr in enumerate(prods):
+                mad_prod(pr, carry=n > 0)
         pend = st.finish(keep=2 if low else 0)
         out.append(render(st, [], False))
         started = st.c2_started
@@ -165,6 +176,18 @@ def emit():
     out.append("  (void)dd;")
     out.append("  return r;")
     out.append("}")
+    return out
+
+
+def emit():
+    out = []
+    out.append("// Generated by gen_mont_asm.py: 8 x 32-bit-limb Montgomery products")
+    out.append("// (finely integrated product scanning) for gfx950.")
+    out.append("// Included by field.cuh inside namespace bc, device compilation only.")
+    out.append("")
+    out += emit_function("mul_lazy_device", [("a", "b")])
+    out.append("")
+    out += emit_function("mul2_lazy_device", [("a", "b"), ("u", "w")])
     return "\n".join(out) + "\n"
 
 
@@ -173,14 +196,17 @@ def check_schedule(text):
     import re
 
     stmts = re.findall(r'asm\("([^"]*)"', text)
-    assert len(stmts) == 23
-    for stmt in stmts:
+    assert len(stmts) == 2 * 23
+    for idx, stmt in enumerate(stmts):
         lines = stmt.split("\\n\\t")
         last_write = {}
         for pos, line in enumerate(lines):
             if line.startswith("v_mad_u64_u32"):
-                pair = re.match(r"v_mad_u64_u32 %\[acc\], %\[(\w+)\]", line).group(1)
+                ops = re.findall(r"%\[(\w+)\]", line)  # acc, carry-out, x, y, acc
+                pair = ops[1]
                 last_write[pair] = pos
+                if pair == "dd":  # carry dropped: an m * N product, or the first product of a function
+                    assert (ops[2].startswith("m") and ops[3].startswith("n")) or (idx % 23 == 0 and pos == 0), line
             elif line.startswith("v_addc"):
                 pair = re.search(r"%\[(p\d)\]$", line).group(1)
                 w = last_write.get(pair, -2)  # written in the previous statement
