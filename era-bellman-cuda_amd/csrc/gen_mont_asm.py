@@ -16,7 +16,7 @@ instructions after the multiply that wrote its pair; the adds write their own
 column is too short to fill the gap does the scheduler emit s_nop.
 
 The first product of each column keeps no carry.  From column 1 on it is an
-m_i * N_j product: every modulus limb is below 0xF0000001, so it is below
+m_i * N_j product: every modulus limb is at most 0xF0000001, so it is below
 2^64 - 2^60, and the carried-in value is below 2^37 (a column holds at most 24
 products), so their sum stays below 2^64.  Column 0 starts from zero.
 
