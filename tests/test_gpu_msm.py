@@ -77,7 +77,7 @@ def test_msm_edge_scalars(bellman, oracle):
 
 
 def test_msm_config_2_20(bellman, oracle):
-    """BASELINE config: n = 2^20, scalars 70% uniform / 20% < 2^64 / 10% in {0,1}."""
+    """The 2^20 MSM workload: n = 2^20, scalars 70% uniform / 20% < 2^64 / 10% in {0,1}."""
     B, dev = bellman
     L = 20
     n = 1 << L
