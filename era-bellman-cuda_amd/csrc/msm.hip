@@ -171,23 +171,56 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
 
 // 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted entries
 // (window w of a job is the segment [entry_begin + w n, entry_begin + (w+1) n))
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t entries,
-                                                                uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
-  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= entries)
-    return;
-  const job_desc &jd = jobs.job[find_job(jobs, e, by_entry())];
-  const uint32_t k = (uint32_t)sorted[e];
+constexpr unsigned RUN_ITEMS = 8; // entries per thread of bucket_runs_kernel
+
+__device__ __forceinline__ void bucket_run_ends(const job_desc &jd, uint32_t e, uint32_t k, uint32_t k_prev, uint32_t k_next, uint32_t *start,
+                                                uint32_t *end) {
   const uint32_t top = 1u << (jd.c - 1);
   if (k == top)
     return; // zero digits
   const uint32_t local = e - (uint32_t)jd.entry_begin, w = local >> jd.log_n;
   const uint32_t seg_begin = (uint32_t)jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
   const uint32_t b = jd.bucket_begin + w * top + k;
-  if (e == seg_begin || (uint32_t)sorted[e - 1] != k)
+  if (e == seg_begin || k_prev != k)
     start[b] = e;
-  if (e + 1 == seg_end || (uint32_t)sorted[e + 1] != k)
+  if (e + 1 == seg_end || k_next != k)
     end[b] = e + 1;
+}
+
+// A block covers MSM_BLOCK * RUN_ITEMS consecutive entries.  All keys and
+// their neighbours' keys are loaded first (clamped indexes stay in bounds), so
+// a wave waits for memory once; a block inside one job reads that job's
+// descriptor with scalar loads, a block across jobs looks each entry's job up.
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t entries,
+                                                                uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
+  const uint32_t first = blockIdx.x * (MSM_BLOCK * RUN_ITEMS);
+  const uint32_t last = min(first + MSM_BLOCK * RUN_ITEMS, entries) - 1;
+  const uint32_t *keys = reinterpret_cast<const uint32_t *>(sorted); // the low words (little-endian)
+  uint32_t k[RUN_ITEMS], k_prev[RUN_ITEMS], k_next[RUN_ITEMS];
+#pragma unroll
+  for (unsigned s = 0; s < RUN_ITEMS; s++) {
+    const uint32_t e = min(first + s * MSM_BLOCK + threadIdx.x, last);
+    k[s] = keys[2 * (uint64_t)e];
+    k_prev[s] = keys[2 * (uint64_t)(e ? e - 1 : 0)];
+    k_next[s] = keys[2 * (uint64_t)(e + 1 < entries ? e + 1 : e)];
+  }
+  const unsigned j0 = find_job(jobs, first, by_entry()), j1 = find_job(jobs, last, by_entry());
+  if (j0 == j1) {
+    const job_desc &jd = jobs.job[j0];
+#pragma unroll
+    for (unsigned s = 0; s < RUN_ITEMS; s++) {
+      const uint32_t e = first + s * MSM_BLOCK + threadIdx.x;
+      if (e <= last)
+        bucket_run_ends(jd, e, k[s], k_prev[s], k_next[s], start, end);
+    }
+  } else {
+#pragma unroll
+    for (unsigned s = 0; s < RUN_ITEMS; s++) {
+      const uint32_t e = first + s * MSM_BLOCK + threadIdx.x;
+      if (e <= last)
+        bucket_run_ends(jobs.job[find_job(jobs, e, by_entry())], e, k[s], k_prev[s], k_next[s], start, end);
+    }
+  }
 }
 
 // counts[b] = end[b] - start[b] (in place over `end`), pieces of at most `piece` points
@@ -638,7 +671,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     BC_TRY(ar.put(digits));
     BC_TRY(fill_u32(offsets, nb, 0, s));
     BC_TRY(fill_u32(counts, nb, 0, s));
-    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, sorted, (uint32_t)entries, offsets, counts);
+    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK * RUN_ITEMS), MSM_BLOCK, 0, s>>>(jt, sorted, (uint32_t)entries, offsets, counts);
     BC_LAUNCH_CHECK();
   }
 

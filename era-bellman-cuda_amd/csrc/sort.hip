@@ -86,8 +86,20 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_histogram_kernel(const ss_table t
   __syncthreads();
   const ss_tile tl = ss_locate(t, digits);
   const uint32_t mask = digits - 1;
-  for (unsigned i = threadIdx.x; i < tl.len; i += SS_BLOCK)
-    atomicAdd(&hist[((uint32_t)keys[tl.start + i] >> shift) & mask], 1u);
+  const E *src = keys + tl.start + threadIdx.x;
+  if (tl.len == SS_TILE) {
+    // all loads in flight before the first use
+    uint32_t k[SS_ITEMS];
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++)
+      k[s] = (uint32_t)src[s * SS_BLOCK];
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++)
+      atomicAdd(&hist[(k[s] >> shift) & mask], 1u);
+  } else {
+    for (unsigned i = threadIdx.x; i < tl.len; i += SS_BLOCK)
+      atomicAdd(&hist[((uint32_t)keys[tl.start + i] >> shift) & mask], 1u);
+  }
   __syncthreads();
   for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
     table[tl.slot0 + (uint64_t)d * tl.stride] = hist[d];
@@ -108,12 +120,25 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kernel(const ss_table t, 
   const ss_tile tl = ss_locate(t, digits);
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   uint32_t key[SS_ITEMS], val[SS_ITEMS], rank[SS_ITEMS];
+  const uint64_t first = tl.start + wave * (SS_ITEMS * 64) + lane;
+  if (tl.len == SS_TILE) {
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++) {
+      key[s] = keys_in[first + s * 64];
+      val[s] = WITH_VALUES ? vals_in[first + s * 64] : 0u;
+    }
+  } else {
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++) {
+      const bool valid = wave * (SS_ITEMS * 64) + s * 64 + lane < tl.len;
+      key[s] = valid ? keys_in[first + s * 64] : 0u;
+      val[s] = (WITH_VALUES && valid) ? vals_in[first + s * 64] : 0u;
+    }
+  }
 #pragma unroll
   for (unsigned s = 0; s < SS_ITEMS; s++) {
     const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
     const bool valid = i < tl.len;
-    key[s] = valid ? keys_in[tl.start + i] : 0u;
-    val[s] = (WITH_VALUES && valid) ? vals_in[tl.start + i] : 0u;
     const uint32_t d = (key[s] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (unsigned b = 0; b < bits; b++) {
@@ -161,19 +186,30 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kv_kernel(const ss_table 
   const unsigned digits = 1u << bits;
   const uint32_t mask = digits - 1;
   const unsigned lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const ss_tile tl = ss_locate(t, digits);
+  // the tile's loads are all issued before the histogram is cleared: a wave
+  // then waits for HBM once, not once per 64 entries
+  uint64_t item[SS_ITEMS];
+  const uint64_t *src = in + tl.start + wave * (SS_ITEMS * 64) + lane;
+  if (tl.len == SS_TILE) {
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++)
+      item[s] = src[s * 64];
+  } else {
+#pragma unroll
+    for (unsigned s = 0; s < SS_ITEMS; s++)
+      item[s] = wave * (SS_ITEMS * 64) + s * 64 + lane < tl.len ? src[s * 64] : 0ull;
+  }
   for (unsigned d = threadIdx.x; d < digits; d += SS_BLOCK)
     for (unsigned w = 0; w < SS_WAVES; w++)
       wave_hist[w][d] = 0;
   __syncthreads();
-  const ss_tile tl = ss_locate(t, digits);
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
-  uint64_t item[SS_ITEMS];
   uint32_t rank[SS_ITEMS];
 #pragma unroll
   for (unsigned s = 0; s < SS_ITEMS; s++) {
     const uint32_t i = wave * (SS_ITEMS * 64) + s * 64 + lane;
     const bool valid = i < tl.len;
-    item[s] = valid ? in[tl.start + i] : 0ull;
     const uint32_t d = ((uint32_t)item[s] >> shift) & mask;
     uint64_t peers = __ballot(valid);
     for (unsigned b = 0; b < bits; b++) {
