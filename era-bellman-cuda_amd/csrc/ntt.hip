@@ -17,11 +17,12 @@
 // both orders: natural->bit-reversed runs Cooley-Tukey butterflies over
 // s = 0..L-1; bit-reversed->natural runs the transposed network, Gentleman-
 // Sande butterflies (E,O) -> (E+O, w(E-O)) over s = L-1..0.  A pass executes a
-// run of consecutive stages [s_lo, s_hi) on LDS tiles of up to 4096 elements
-// (128 KiB): the stages' butterflies stay inside groups of 2^(s_hi-s_lo)
-// elements, strided by 2^(L-s_hi).  log n <= 24 takes at most two HBM passes.
-// Tiles are mapped XCD-major so that neighbouring strided columns run on the
-// same XCD and share its L2 lines.
+// run of at most NTT_LOG_TILE consecutive stages [s_lo, s_hi) on LDS tiles of
+// 2^NTT_LOG_TILE = 1024 elements (32 KiB, five blocks per CU): the stages'
+// butterflies stay inside groups of 2^(s_hi-s_lo) elements, strided by
+// 2^(L-s_hi), so a transform of 2^L takes ceil(L / 10) HBM passes (three at
+// 2^24).  Tiles are mapped XCD-major so that neighbouring strided columns run
+// on the same XCD and share its L2 lines.
 #include <algorithm>
 #include <mutex>
 #include <vector>
@@ -130,13 +131,31 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
   const unsigned elems = 1u << (log_g + log_t + log_c);
   const unsigned cmask = (1u << log_c) - 1, tmask = (1u << log_t) - 1;
 
-  for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK) {
+  // element e of the tile <-> index of the transform
+  auto index_of = [&](unsigned e) {
     const unsigned c = e & cmask, t = (e >> log_c) & tmask, g = e >> (log_c + log_t);
-    const unsigned idx = ((hi0 + g) << (L - s_lo)) | (t << log_lo) | (lo0 + c);
-    fe v = load_fe(a.in + idx);
-    if (a.pre_table)
-      v = mul_lazy<F>(v, twist_pow(a.aux, GS ? brev_bits(idx, L) : idx));
-    tile[e] = v;
+    return ((hi0 + g) << (L - s_lo)) | (t << log_lo) | (lo0 + c);
+  };
+  constexpr unsigned PER_THREAD = (1u << NTT_LOG_TILE) / NTT_BLOCK;
+  if (a.pre_table) {
+    // coset twist: one element at a time (the product's operands then never sit
+    // in a register tuple shared with other loads in flight)
+    for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK) {
+      const unsigned idx = index_of(e);
+      tile[e] = mul_lazy<F>(load_fe(a.in + idx), twist_pow(a.aux, GS ? brev_bits(idx, L) : idx));
+    }
+  } else if (elems == (1u << NTT_LOG_TILE)) {
+    // a thread's loads are all issued before the first is used
+    fe v[PER_THREAD];
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++)
+      v[r] = load_fe(a.in + index_of(threadIdx.x + r * NTT_BLOCK));
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++)
+      tile[threadIdx.x + r * NTT_BLOCK] = v[r];
+  } else {
+    for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK)
+      tile[e] = load_fe(a.in + index_of(e));
   }
   __syncthreads();
 
@@ -188,12 +207,14 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
       const unsigned i1 = i0 + (hh << log_c), i2 = i0 + (h << log_c), i3 = i2 + (hh << log_c);
       const unsigned b = ((hi0 + g) << (s - s_lo)) | (t0 >> (s_hi - s));
       const fe *tw_s1 = tw + (2u << s) + 2 * b; // stage s+1 twiddles of blocks 2b, 2b+1
+      // the step's twiddles are loaded ahead of the tile reads (for s = 0,
+      // TW[1] = 1 is loaded and not used)
+      const fe wa = load_fe(tw + (1u << s) + b), wb0 = load_fe(tw_s1), wb1 = load_fe(tw_s1 + 1);
       fe x0 = tile[i0], x1 = tile[i1], x2 = tile[i2], x3 = tile[i3];
       if (!GS) {
         // values < 4m in and out: the odd inputs of a butterfly go through a
         // twiddle product (< 2m), the even ones through one reduction (< 2m)
         if (s != 0) {
-          const fe wa = load_fe(tw + (1u << s) + b);
           x2 = mul_lazy<F>(x2, wa);
           x3 = mul_lazy<F>(x3, wa);
         } else {
@@ -203,19 +224,18 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
         x0 = reduce2<F>(x0);
         x1 = reduce2<F>(x1);
         const fe y0 = reduce2<F>(add_raw(x0, x2)), y2 = reduce2<F>(sub_raw2<F>(x0, x2));
-        const fe y1 = mul_lazy<F>(add_raw(x1, x3), load_fe(tw_s1));
-        const fe y3 = mul_lazy<F>(sub_raw2<F>(x1, x3), load_fe(tw_s1 + 1));
+        const fe y1 = mul_lazy<F>(add_raw(x1, x3), wb0);
+        const fe y3 = mul_lazy<F>(sub_raw2<F>(x1, x3), wb1);
         tile[i0] = add_raw(y0, y1);
         tile[i1] = sub_raw2<F>(y0, y1);
         tile[i2] = add_raw(y2, y3);
         tile[i3] = sub_raw2<F>(y2, y3);
       } else {
         // values < 2m in and out; differences that feed a product skip the reduction
-        const fe y0 = add2<F>(x0, x1), y1 = mul_lazy<F>(sub_raw2<F>(x0, x1), load_fe(tw_s1));
-        const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub_raw2<F>(x2, x3), load_fe(tw_s1 + 1));
+        const fe y0 = add2<F>(x0, x1), y1 = mul_lazy<F>(sub_raw2<F>(x0, x1), wb0);
+        const fe y2 = add2<F>(x2, x3), y3 = mul_lazy<F>(sub_raw2<F>(x2, x3), wb1);
         fe z2, z3;
         if (s != 0) {
-          const fe wa = load_fe(tw + (1u << s) + b);
           z2 = mul_lazy<F>(sub_raw2<F>(y0, y2), wa);
           z3 = mul_lazy<F>(sub_raw2<F>(y1, y3), wa);
         } else {
