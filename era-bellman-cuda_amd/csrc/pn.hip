@@ -10,7 +10,6 @@
 #include "batch.h"
 #include "common.h"
 #include "pn.h"
-#include "scan.h"
 #include "sort.h"
 #include "tables.h"
 
@@ -32,31 +31,35 @@ __global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, 
   entries[p] = (uint64_t)cell << 32 | indexes[cell];
 }
 
-__global__ void pn_run_flags_kernel(const uint64_t *sorted, uint32_t *flags, unsigned n) {
-  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n)
-    flags[j] = (j == 0 || (uint32_t)sorted[j] != (uint32_t)sorted[j - 1]) ? 1u : 0u;
-}
-
-// run r (r = excl[j] + flags[j] - 1 at sorted position j) ends at run_end[r]
-__global__ void pn_run_ends_kernel(const uint32_t *flags, const uint32_t *excl, uint32_t *run_end, unsigned n) {
-  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n && (j == n - 1 || flags[j + 1]))
-    run_end[excl[j] + flags[j] - 1] = j + 1;
-}
-
-// Sorted position j of a run of length len at offset o sends its value to
-// position o + ((j - o + len - 1) mod len): the previous position, or the
-// run's last one for its first position; run 0 maps to itself
-// (src/pn_kernels.cu:89-121).
-__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint64_t *sorted, const uint32_t *flags, const uint32_t *excl,
-                                   const uint32_t *run_end, fe *target, unsigned n, unsigned log_rows) {
-  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n)
+// start_of[v] = the first sorted position of variable v's run
+__global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of, unsigned n, uint32_t key_mask) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n)
     return;
-  const uint32_t dst = j < run_end[0] ? j : (flags[j] ? run_end[excl[j]] - 1 : j - 1);
-  const uint32_t src_cell = (uint32_t)(sorted[j] >> 32);
-  const uint32_t dst_cell = (uint32_t)(sorted[dst] >> 32);
+  const uint32_t *keys = reinterpret_cast<const uint32_t *>(sorted); // the low words (little-endian)
+  const uint32_t k = keys[2 * (uint64_t)p], k_prev = keys[2 * (uint64_t)(p ? p - 1 : 0)];
+  if (p == 0 || k != k_prev)
+    start_of[k & key_mask] = p;
+}
+
+// In the reference's direction (src/pn_kernels.cu:89-121), sorted position j
+// of a run sends its value to the previous position, or to the run's last
+// position when j is the run's first; the first run maps to itself.  Here each
+// sorted position p gathers instead: from p + 1 inside its run, from the run's
+// first position when p is the run's last (one table read per run), from
+// itself in the first run, and writes target[cell(p)] = omega^row * k_col of
+// that source cell.  A thread's loads are all independent but the last.
+__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint64_t *sorted, const uint32_t *start_of, fe *target, unsigned n,
+                                   unsigned log_rows, uint32_t key_mask) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n)
+    return;
+  const uint64_t here = sorted[p], next = sorted[p + 1 < n ? p + 1 : p];
+  const uint32_t k = (uint32_t)here, k_first = (uint32_t)sorted[0];
+  uint64_t src = here; // the first run maps to itself
+  if (k != k_first)
+    src = p + 1 < n && (uint32_t)next == k ? next : sorted[min(start_of[k & key_mask], n - 1)]; // clamped for indexes >= 2^bits
+  const uint32_t src_cell = (uint32_t)(src >> 32), dst_cell = (uint32_t)(here >> 32);
   const uint32_t row = src_cell & ((1u << log_rows) - 1), col = src_cell >> log_rows;
   const fe tw = table_pow(w, (row << (POW_LOG_RANGE - log_rows)) & ((1u << POW_LOG_RANGE) - 1));
   store_fe(target + dst_cell, mul<FrParams>(tw, load_fe(scalars + col)));
@@ -75,7 +78,6 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   const unsigned n = cols << L;
   const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
   uint64_t *sorted = nullptr, *entries = nullptr;
-  uint32_t *flags = nullptr, *excl = nullptr, *run_end = nullptr;
   BC_TRY(ar.get(&sorted, n));
   BC_TRY(ar.get(&entries, n)); // on top of the stack: reclaimed after the sort
   pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, entries, cols, L);
@@ -83,18 +85,15 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   const seg_group all{0, n, 1};
   BC_TRY(segmented_sort_kv(entries, sorted, &all, 1, bits, ar));
   BC_TRY(ar.put(entries));
-  BC_TRY(ar.get(&flags, n));
-  BC_TRY(ar.get(&excl, n));
-  pn_run_flags_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, flags, n);
+  // keys are sorted on their low `bits` bits: variable indexes below 2^bits
+  const uint32_t key_mask = (1u << bits) - 1;
+  uint32_t *start_of = nullptr;
+  BC_TRY(ar.get(&start_of, (size_t)key_mask + 1));
+  pn_run_starts_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, start_of, n, key_mask);
   BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(flags, excl, n, ar));
-  BC_TRY(ar.get(&run_end, n));
-  pn_run_ends_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(flags, excl, run_end, n);
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, sorted, start_of, target, n, L, key_mask);
   BC_LAUNCH_CHECK();
-  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, sorted, flags, excl, run_end, target, n, L);
-  BC_LAUNCH_CHECK();
-  for (uint32_t *p : {run_end, excl, flags})
-    BC_TRY(ar.put(p));
+  BC_TRY(ar.put(start_of));
   BC_TRY(ar.put(sorted));
   return hipSuccess;
 }

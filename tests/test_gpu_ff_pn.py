@@ -237,6 +237,31 @@ def test_pn_random_cycles(bellman, oracle, cols, log_rows):
     assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), oracle.pn(idx, ks, cols, log_rows))
 
 
+@pytest.mark.parametrize("cols,log_rows", [(3, 8), (4, 11)])
+def test_pn_sparse_variable_indexes(bellman, oracle, cols, log_rows):
+    """Variable indexes with gaps anywhere below 2^(2 + log_rows), the smallest
+    not 0: three one-cell cycles, one cycle over a third of the cells, then
+    random lengths."""
+    _, dev = bellman
+    cells = cols << log_rows
+    rng = random.Random(7 * cells + 1)
+    perm = list(range(cells))
+    rng.shuffle(perm)
+    keys = rng.sample(range(1, 4 << log_rows), cells // 4)
+    idx = np.zeros(cells, dtype=np.uint32)
+    pos = v = 0
+    while pos < cells:
+        L = [1, 1, 1, cells // 3][v] if v < 4 else rng.randint(1, 40)
+        if v == len(keys) - 1:
+            L = cells - pos
+        idx[perm[pos:pos + L]] = keys[v]
+        pos += L
+        v += 1
+    assert idx.min() > 0
+    ks = oracle.random_fr(cells + 1, cols)
+    assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), oracle.pn(idx, ks, cols, log_rows))
+
+
 def test_pn_rejects_other_column_counts(bellman):
     _, dev = bellman
     cfg = B.generate_permutation_polynomials_configuration(dev.pool, dev.stream, None, None, None, 5, 4)
