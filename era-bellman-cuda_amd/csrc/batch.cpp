@@ -31,15 +31,15 @@ constexpr unsigned TUNE_MAX_LOG = 28;
 // than the plain call's pay off in batches: fewer buckets to reduce per job.
 msm_tuning msm_builtin_tuning(unsigned l) {
   static const msm_tuning measured[] = {
-      {5, 32},  /* 12 */ {5, 16},  /* 13 */ {5, 32},  /* 14 */ {7, 32},  /* 15 */ {9, 32},  /* 16 */
-      {9, 32},  /* 17 */ {11, 16}, /* 18 */ {11, 32}, /* 19 */ {13, 16}, /* 20 */ {14, 16}, /* 21 */
-      {15, 32}, /* 22 */ {15, 32}, /* 23 */ {17, 32}, /* 24 */
+      {5, 32},  /* 12 */ {9, 8},   /* 13 */ {7, 32},  /* 14 */ {11, 16}, /* 15 */ {11, 8},   /* 16 */
+      {13, 16}, /* 17 */ {13, 32}, /* 18 */ {15, 32}, /* 19 */ {15, 16}, /* 20 */ {17, 32},  /* 21 */
+      {17, 32}, /* 22 */ {17, 64}, /* 23 */ {17, 128}, /* 24 */
   };
   if (l < 12)
-    return {std::max(3u, l > 7 ? l - 7 : 3u), 16};
+    return {std::min(5u, l > 8 ? l - 5 : 3u), 16};
   if (l <= 24)
     return measured[l - 12];
-  return {std::min(msm_window_bits(l), 22u), 32};
+  return {std::min(msm_window_bits(l), 22u), 128};
 }
 
 struct tuning_table {
@@ -95,7 +95,7 @@ hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out) {
   out->log_n = in.log_scalars_count;
   out->c = in.window_bits ? in.window_bits : t.c;
   out->piece = in.piece ? in.piece : t.piece;
-  if (out->c < 3 || out->c > 24)
+  if (out->c < 3 || out->c > 24 || out->piece > MSM_MAX_PIECE)
     return hipErrorInvalidValue;
   return hipSuccess;
 }
@@ -491,7 +491,7 @@ bc_error bc_amd_msm_get_tuning(unsigned log_scalars_count, bc_amd_msm_tuning *tu
 }
 
 bc_error bc_amd_msm_set_tuning(unsigned log_scalars_count, bc_amd_msm_tuning tuning) {
-  if (log_scalars_count > 28 || tuning.window_bits < 3 || tuning.window_bits > 24 || tuning.piece == 0)
+  if (log_scalars_count > 28 || tuning.window_bits < 3 || tuning.window_bits > 24 || tuning.piece == 0 || tuning.piece > MSM_MAX_PIECE)
     return bc_error_invalid_value;
   msm_set_tuned(log_scalars_count, {tuning.window_bits, tuning.piece});
   return bc_success;

@@ -84,15 +84,17 @@ constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many pieces are s
 struct job_desc {
   const g1_affine *bases;
   const fe *scalars;
-  uint64_t entry_begin; // first (key, value) pair of the job
-  uint32_t n;           // scalars
-  uint32_t log_n;
-  uint32_t c;           // window bits
-  uint32_t W;           // windows
+  uint32_t entry_begin;  // first (key, value) pair of the job (a launch holds < 2^31)
+  uint32_t n;            // scalars
   uint32_t scalar_begin;
   uint32_t bucket_begin; // first batch-global bucket
   uint32_t top_begin;    // first top-digit slot
-  uint32_t piece;        // most points per accumulating thread
+  uint32_t chunk_begin;  // first accumulation chunk of the job
+  uint32_t slot_begin;   // first partial-sum slot of the job
+  uint16_t piece;        // sorted entries per accumulation chunk
+  uint8_t log_n;
+  uint8_t c; // window bits
+  uint8_t W; // windows
 };
 
 static_assert(MSM_MAX_JOBS <= SEG_MAX_GROUPS, "one sort segment group per job");
@@ -123,7 +125,10 @@ struct by_bucket {
   __device__ uint32_t operator()(const job_desc &d) const { return d.bucket_begin; }
 };
 struct by_entry {
-  __device__ uint32_t operator()(const job_desc &d) const { return (uint32_t)d.entry_begin; }
+  __device__ uint32_t operator()(const job_desc &d) const { return d.entry_begin; }
+};
+struct by_chunk {
+  __device__ uint32_t operator()(const job_desc &d) const { return d.chunk_begin; }
 };
 
 __global__ void fill_u32_kernel(uint32_t *p, size_t n, uint32_t v) {
@@ -178,8 +183,8 @@ __device__ __forceinline__ void bucket_run_ends(const job_desc &jd, uint32_t e, 
   const uint32_t top = 1u << (jd.c - 1);
   if (k == top)
     return; // zero digits
-  const uint32_t local = e - (uint32_t)jd.entry_begin, w = local >> jd.log_n;
-  const uint32_t seg_begin = (uint32_t)jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
+  const uint32_t local = e - jd.entry_begin, w = local >> jd.log_n;
+  const uint32_t seg_begin = jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
   const uint32_t b = jd.bucket_begin + w * top + k;
   if (e == seg_begin || k_prev != k)
     start[b] = e;
@@ -223,79 +228,74 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table 
   }
 }
 
-// counts[b] = end[b] - start[b] (in place over `end`), pieces of at most `piece` points
-__global__ void piece_count_kernel(const job_table jobs, const uint32_t *start, uint32_t *counts, uint32_t *pieces, unsigned nb) {
+// 3. accumulation over equal chunks of the sorted entries.  Chunk k of a job
+// is its entries [k piece, (k+1) piece); one thread adds the chunk's points
+// into a running XYZZ sum and stores the sum at slot(b, k) = slot_begin +
+// (b - bucket_begin) + k whenever the bucket b it was adding to ends inside the
+// chunk, and at the chunk's end.  Along the sorted entries neither b nor k
+// decreases, so every (chunk, bucket) pair has a slot of its own and a
+// bucket's sums occupy consecutive slots.  All threads of a wave do the same
+// number of additions whatever the bucket sizes (cutting buckets into pieces
+// instead leaves a wave waiting for its longest piece).
+
+// piece_offset[b], piece_count[b]: the slots of bucket b's sums, from its run
+// [start, end) of sorted entries (empty buckets have none)
+__global__ void bucket_slots_kernel(const job_table jobs, const uint32_t *start, const uint32_t *end, uint32_t *piece_offset, uint32_t *piece_count,
+                                    unsigned nb) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb)
     return;
-  const uint32_t piece = jobs.job[find_job(jobs, b, by_bucket())].piece;
-  const uint32_t cnt = counts[b] - start[b];
-  counts[b] = cnt;
-  pieces[b] = (cnt + piece - 1) / piece;
-}
-
-// piece -> bucket map: one thread writes a bucket's first PIECE_MAP_DIRECT
-// entries; buckets with more pieces (a few giant buckets, e.g. the last window
-// when it keeps only 2 bits) are listed and filled by piece_map_big_kernel.
-constexpr unsigned PIECE_MAP_DIRECT = 64;
-
-__global__ void piece_map_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket, unsigned nb, uint32_t *total,
-                                 uint32_t *big, uint32_t *big_count) {
-  const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb)
-    return;
-  const uint32_t np = piece_count[b], off = piece_offset[b];
-  const uint32_t direct = np < PIECE_MAP_DIRECT ? np : PIECE_MAP_DIRECT;
-  for (uint32_t q = 0; q < direct; q++)
-    piece_bucket[off + q] = b;
-  if (np > PIECE_MAP_DIRECT)
-    big[atomicAdd(big_count, 1u)] = b;
-  if (b == nb - 1)
-    *total = off + np;
-}
-
-__global__ __launch_bounds__(256) void piece_map_big_kernel(const uint32_t *piece_count, const uint32_t *piece_offset, uint32_t *piece_bucket,
-                                                            const uint32_t *big, const uint32_t *big_count) {
-  const uint32_t count = *big_count;
-  for (uint32_t i = blockIdx.x; i < count; i += gridDim.x) {
-    const uint32_t b = big[i], np = piece_count[b], off = piece_offset[b];
-    for (uint32_t q = PIECE_MAP_DIRECT + threadIdx.x; q < np; q += blockDim.x)
-      piece_bucket[off + q] = b;
-  }
-}
-
-// 3. one thread accumulates one piece (<= piece points of one bucket)
-__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, const uint32_t *__restrict__ offsets,
-                                                               const uint32_t *__restrict__ counts, const uint32_t *__restrict__ piece_offset,
-                                                               const uint32_t *__restrict__ piece_bucket, const uint32_t *__restrict__ total,
-                                                               g1_xyzz *__restrict__ partials) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= *total)
-    return;
-  const uint32_t b = piece_bucket[p];
   const job_desc &jd = jobs.job[find_job(jobs, b, by_bucket())];
+  const uint32_t s0 = start[b], e0 = end[b];
+  if (e0 <= s0) {
+    piece_count[b] = 0;
+    piece_offset[b] = 0;
+    return;
+  }
+  const uint32_t k0 = (s0 - jd.entry_begin) / jd.piece, k1 = (e0 - 1 - jd.entry_begin) / jd.piece;
+  piece_offset[b] = jd.slot_begin + (b - jd.bucket_begin) + k0;
+  piece_count[b] = k1 - k0 + 1;
+}
+
+__global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t chunks,
+                                                               g1_xyzz *__restrict__ partials) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= chunks)
+    return;
+  const job_desc &jd = jobs.job[find_job(jobs, t, by_chunk())];
   const g1_affine *__restrict__ bases = jd.bases;
-  const uint32_t piece = jd.piece;
-  const uint32_t q = p - piece_offset[b];
-  const uint32_t begin = offsets[b] + q * piece;
-  const uint32_t stop = offsets[b] + counts[b];
-  const uint32_t end = begin + piece < stop ? begin + piece : stop;
+  const uint32_t k = t - jd.chunk_begin, top = 1u << (jd.c - 1);
+  const uint32_t begin = jd.entry_begin + k * jd.piece;
+  const uint32_t job_end = jd.entry_begin + ((uint32_t)jd.W << jd.log_n);
+  const uint32_t end = begin + jd.piece < job_end ? begin + jd.piece : job_end;
+  g1_xyzz *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
+  constexpr uint32_t NONE = 0xffffffffu;
+  uint32_t cur = NONE;
   g1_xyzz acc = xyzz_infinity();
-  if (begin < end) {
-    // the next point's loads are issued before the current addition
-    uint32_t ref = (uint32_t)(sorted[begin] >> 32);
-    g1_affine pt = load_affine(bases + (ref & ~SIGN));
-    for (uint32_t e = begin; e < end; e++) {
-      const uint32_t ref_next = e + 1 < end ? (uint32_t)(sorted[e + 1] >> 32) : ref;
-      const g1_affine next = load_affine(bases + (ref_next & ~SIGN));
-      if (ref & SIGN)
+  // the next entry's point is loaded before the current addition
+  uint64_t ent = sorted[begin];
+  g1_affine pt = load_affine(bases + ((uint32_t)(ent >> 32) & ~SIGN));
+  for (uint32_t e = begin; e < end; e++) {
+    const uint64_t ent_next = e + 1 < end ? sorted[e + 1] : ent;
+    const g1_affine next = load_affine(bases + ((uint32_t)(ent_next >> 32) & ~SIGN));
+    const uint32_t key = (uint32_t)ent;
+    if (key != top) { // zero digits add nothing
+      const uint32_t b = jd.bucket_begin + ((e - jd.entry_begin) >> jd.log_n) * top + key;
+      if (b != cur) {
+        if (cur != NONE)
+          store_xyzz(slots + cur, acc);
+        acc = xyzz_infinity();
+        cur = b;
+      }
+      if ((uint32_t)(ent >> 32) & SIGN)
         pt.y = neg2<Fq>(pt.y);
       acc = xyzz_add_affine(acc, pt);
-      pt = next;
-      ref = ref_next;
     }
+    pt = next;
+    ent = ent_next;
   }
-  store_xyzz(partials + p, acc);
+  if (cur != NONE)
+    store_xyzz(slots + cur, acc);
 }
 
 // 4a. buckets of 2..SMALL_PIECES pieces are summed serially into piece 0;
@@ -618,35 +618,39 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     return hipErrorInvalidValue;
   job_table jt{};
   jt.count = count;
-  uint64_t entries = 0, nb64 = 0;
+  uint64_t entries = 0, nb64 = 0, chunks = 0, slots = 0;
   uint32_t scalars_total = 0, tops_total = 0, max_np = 0;
-  uint64_t max_pieces = 1;
   for (unsigned j = 0; j < count; j++) {
     const unsigned c = jobs[j].c;
-    if (c < 3 || c > 24 || jobs[j].log_n > 28 || jobs[j].piece == 0 || !jobs[j].bases || !jobs[j].scalars || !jobs[j].results)
+    if (c < 3 || c > 24 || jobs[j].log_n > 28 || jobs[j].piece == 0 || jobs[j].piece > MSM_MAX_PIECE || !jobs[j].bases || !jobs[j].scalars ||
+        !jobs[j].results)
       return hipErrorInvalidValue;
     job_desc &d = jt.job[j];
     d.bases = jobs[j].bases;
     d.scalars = jobs[j].scalars;
     d.n = 1u << jobs[j].log_n;
-    d.log_n = jobs[j].log_n;
-    d.c = c;
-    d.W = (254 - 1) / c + 1;
+    d.log_n = (uint8_t)jobs[j].log_n;
+    d.c = (uint8_t)c;
+    d.W = (uint8_t)((254 - 1) / c + 1);
+    d.piece = (uint16_t)jobs[j].piece;
+    if ((uint64_t)scalars_total + d.n > 0xffffffffull || entries + (uint64_t)d.W * d.n >= (1ull << 31))
+      return hipErrorInvalidValue;
     d.scalar_begin = scalars_total;
-    d.entry_begin = entries;
+    d.entry_begin = (uint32_t)entries;
     d.bucket_begin = (uint32_t)nb64;
     d.top_begin = tops_total;
-    d.piece = jobs[j].piece;
-    if ((uint64_t)scalars_total + d.n > 0xffffffffull)
-      return hipErrorInvalidValue;
+    d.chunk_begin = (uint32_t)chunks;
+    d.slot_begin = (uint32_t)slots;
+    const uint64_t job_chunks = ((uint64_t)d.W * d.n + d.piece - 1) / d.piece, job_buckets = (uint64_t)d.W << (c - 1);
     scalars_total += d.n;
     entries += (uint64_t)d.W * d.n;
-    nb64 += (uint64_t)d.W << (c - 1);
+    nb64 += job_buckets;
     tops_total += d.W;
-    max_np = std::max(max_np, (d.n + d.piece - 1) / d.piece);
-    max_pieces += ((uint64_t)d.W * d.n + d.piece - 1) / d.piece + ((uint64_t)d.W << (c - 1)); // sum of ceil(count/piece) over buckets
+    chunks += job_chunks;
+    slots += job_buckets + job_chunks; // slot(b, k) <= (buckets - 1) + (chunks - 1)
+    max_np = std::max<uint32_t>(max_np, (d.n + d.piece - 1) / d.piece + 1); // a bucket holds at most n entries
   }
-  if (entries >= (1ull << 31) || nb64 >= (1ull << 31))
+  if (entries >= (1ull << 31) || nb64 >= (1ull << 31) || slots >= (1ull << 32))
     return hipErrorInvalidValue;
   const uint32_t nb = (uint32_t)nb64;
 
@@ -665,7 +669,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     unsigned key_bits = 0; // local keys are <= 2^(c-1)
     for (unsigned j = 0; j < count; j++) {
       groups[j] = {jt.job[j].entry_begin, jt.job[j].n, jt.job[j].W};
-      key_bits = std::max(key_bits, jt.job[j].c);
+      key_bits = std::max<unsigned>(key_bits, jt.job[j].c);
     }
     BC_TRY(segmented_sort_kv(digits, sorted, groups, count, key_bits, ar));
     BC_TRY(ar.put(digits));
@@ -675,25 +679,17 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     BC_LAUNCH_CHECK();
   }
 
-  // 3. pieces and accumulation; meta = {piece total, heavy count, heavy chunk total, big count}
-  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *piece_bucket = nullptr, *meta = nullptr, *big = nullptr;
+  // 3. accumulation over chunks of the sorted entries; meta = {heavy count, heavy chunk total}
+  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *meta = nullptr;
   g1_xyzz *partials = nullptr;
   BC_TRY(ar.get(&piece_count, nb));
   BC_TRY(ar.get(&piece_offset, nb));
-  BC_TRY(ar.get(&piece_bucket, max_pieces));
-  BC_TRY(ar.get(&meta, 4));
-  BC_TRY(ar.get(&big, nb));
-  BC_TRY(ar.get(&partials, max_pieces));
-  BC_TRY(fill_u32(meta, 4, 0, s));
-  piece_count_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_count, nb);
+  BC_TRY(ar.get(&meta, 2));
+  BC_TRY(ar.get(&partials, slots));
+  BC_TRY(fill_u32(meta, 2, 0, s));
+  bucket_slots_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_offset, piece_count, nb);
   BC_LAUNCH_CHECK();
-  BC_TRY(scan_u32_exclusive(piece_count, piece_offset, nb, ar));
-  piece_map_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(piece_count, piece_offset, piece_bucket, nb, meta, big, meta + 3);
-  BC_LAUNCH_CHECK();
-  piece_map_big_kernel<<<256, 256, 0, s>>>(piece_count, piece_offset, piece_bucket, big, meta + 3);
-  BC_LAUNCH_CHECK();
-  BC_TRY(ar.put(big));
-  accumulate_kernel<<<grid_for(max_pieces, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, offsets, counts, piece_offset, piece_bucket, meta, partials);
+  accumulate_kernel<<<grid_for(chunks, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, (uint32_t)chunks, partials);
   BC_LAUNCH_CHECK();
 
   // 4. fix-up of multi-piece buckets
@@ -701,14 +697,14 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
     uint32_t *heavy = nullptr, *chunk_offset = nullptr;
     BC_TRY(ar.get(&heavy, nb));
     BC_TRY(ar.get(&chunk_offset, nb));
-    small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta + 1);
+    small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta);
     BC_LAUNCH_CHECK();
     if (max_np > SMALL_PIECES) {
-      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta + 1, piece_count, chunk_offset, meta + 2);
+      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta, piece_count, chunk_offset, meta + 1);
       BC_LAUNCH_CHECK();
-      heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta + 1, chunk_offset, meta + 2, piece_offset, piece_count);
+      heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, chunk_offset, meta + 1, piece_offset, piece_count);
       BC_LAUNCH_CHECK();
-      heavy_final_kernel<<<256, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta + 1, piece_offset, piece_count);
+      heavy_final_kernel<<<256, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, piece_offset, piece_count);
       BC_LAUNCH_CHECK();
     }
     BC_TRY(ar.put(heavy));
@@ -720,7 +716,7 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   BC_TRY(ar.get(&tops, tops_total));
   bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops);
   BC_LAUNCH_CHECK();
-  for (void *p : {(void *)partials, (void *)piece_bucket, (void *)piece_offset, (void *)piece_count, (void *)meta, (void *)offsets, (void *)counts,
+  for (void *p : {(void *)partials, (void *)piece_offset, (void *)piece_count, (void *)meta, (void *)offsets, (void *)counts,
                   (void *)sorted})
     BC_TRY(ar.put(p));
 

@@ -41,7 +41,7 @@ typedef struct {
   void *results;       /* 254 Jacobian points */
   unsigned log_scalars_count;
   unsigned window_bits; /* 0: the tuned width for the size; else 3..24 */
-  unsigned piece;       /* 0: tuned; else the most points one thread accumulates */
+  unsigned piece;       /* 0: tuned; else 1..65535 sorted (digit, point) entries per accumulating thread */
 } bc_amd_msm_job;
 
 /* One transform with the ntt_configuration contract. */
@@ -73,11 +73,11 @@ bc_error bc_amd_msm_batch_async(const bc_amd_msm_job *jobs, unsigned count, bc_m
 bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_mem_pool pool, bc_stream stream);
 
 /* ------------------------------------------------------------------------ */
-/* Tuning (process-wide).  MSM: window width and piece size per log size;    */
-/* NTT: stages of the first HBM pass for sizes of two or more passes (log    */
-/* sizes 11..28, at most 10 stages per pass; the other passes share the rest */
-/* evenly; 0 = balanced).  Tuning changes no folded MSM point and no NTT      */
-/* output.                                                                    */
+/* Tuning (process-wide).  MSM: window width and accumulation chunk (sorted */
+/* entries per thread, the job's `piece`) per log size; NTT: stages of the  */
+/* first HBM pass for sizes of two or more passes (log sizes 11..28, at     */
+/* most 10 stages per pass; the other passes share the rest evenly; 0 =     */
+/* balanced).  Tuning changes no folded MSM point and no NTT output.        */
 
 typedef struct {
   unsigned window_bits;

@@ -16,8 +16,9 @@ def B():
 
 
 def test_msm_tuning_builtin_table(B):
-    """A fresh process starts from the built-in table: measured (window, piece)
-    for 2^12..2^24, the plain call's window width (src/msm.cu:12-37) above."""
+    """A fresh process starts from the built-in table: measured (window,
+    entries per accumulating thread) for 2^12..2^24, the plain call's window
+    width (src/msm.cu:12-37) above."""
     import subprocess
     import sys
 
@@ -27,10 +28,11 @@ def test_msm_tuning_builtin_table(B):
     pkg = os.path.dirname(os.path.dirname(B.__file__))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, env=dict(os.environ, PYTHONPATH=pkg))
     table = {int(a): (int(b), int(c)) for a, b, c in (line.split() for line in out.stdout.splitlines())}
-    assert table[12] == (5, 32) and table[16] == (9, 32) and table[20] == (13, 16) and table[24] == (17, 32)
-    assert table[25] == (20, 32) and table[26] == (22, 32) and table[28] == (22, 32)
+    assert table[12] == (5, 32) and table[16] == (11, 8) and table[20] == (15, 16) and table[24] == (17, 128)
+    assert table[10] == (5, 16) and table[9] == (4, 16) and table[0] == (3, 16)
+    assert table[25] == (20, 128) and table[26] == (22, 128) and table[28] == (22, 128)
     for L, (c, piece) in table.items():
-        assert 3 <= c <= 22 and piece in (16, 32)
+        assert 3 <= c <= 22 and piece in (8, 16, 32, 64, 128)
         assert (254 - 1) // c + 1 <= 85  # windows of at least 3 bits
 
 
@@ -40,6 +42,7 @@ def test_tuning_validation(B):
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(2, 32)) == B.bc_error_invalid_value
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(25, 32)) == B.bc_error_invalid_value
     assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(10, 0)) == B.bc_error_invalid_value
+    assert lib.bc_amd_msm_set_tuning(ctypes.c_uint(10), B.bc_amd_msm_tuning(10, 1 << 16)) == B.bc_error_invalid_value
     # sizes of two or more passes only; every pass at most 10 stages
     assert lib.bc_amd_ntt_set_split(ctypes.c_uint(10), ctypes.c_uint(5)) == B.bc_error_invalid_value  # one pass
     assert lib.bc_amd_ntt_set_split(ctypes.c_uint(24), ctypes.c_uint(11)) == B.bc_error_invalid_value
