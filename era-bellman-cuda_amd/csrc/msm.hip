@@ -272,13 +272,18 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   constexpr uint32_t NONE = 0xffffffffu;
   uint32_t cur = NONE;
   g1_xyzz acc = xyzz_infinity();
-  // the next entry's point is loaded before the current addition
-  uint64_t ent = sorted[begin];
-  g1_affine pt = load_affine(bases + ((uint32_t)(ent >> 32) & ~SIGN));
+  // software pipeline: entries three ahead, their points two ahead (a zero
+  // digit's point is not loaded); indexes past the chunk repeat its last entry
+  const uint32_t last = end - 1;
+  auto point_of = [&](uint64_t ent, const g1_affine &keep) {
+    return (uint32_t)ent != top ? load_affine(bases + ((uint32_t)(ent >> 32) & ~SIGN)) : keep;
+  };
+  uint64_t ent0 = sorted[begin], ent1 = sorted[min(begin + 1, last)], ent2 = sorted[min(begin + 2, last)];
+  g1_affine pt0 = point_of(ent0, g1_affine{}), pt1 = point_of(ent1, pt0);
   for (uint32_t e = begin; e < end; e++) {
-    const uint64_t ent_next = e + 1 < end ? sorted[e + 1] : ent;
-    const g1_affine next = load_affine(bases + ((uint32_t)(ent_next >> 32) & ~SIGN));
-    const uint32_t key = (uint32_t)ent;
+    const uint64_t ent3 = sorted[min(e + 3, last)];
+    const g1_affine pt2 = point_of(ent2, pt1);
+    const uint32_t key = (uint32_t)ent0;
     if (key != top) { // zero digits add nothing
       const uint32_t b = jd.bucket_begin + ((e - jd.entry_begin) >> jd.log_n) * top + key;
       if (b != cur) {
@@ -287,12 +292,16 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
         acc = xyzz_infinity();
         cur = b;
       }
-      if ((uint32_t)(ent >> 32) & SIGN)
+      g1_affine pt = pt0;
+      if ((uint32_t)(ent0 >> 32) & SIGN)
         pt.y = neg2<Fq>(pt.y);
       acc = xyzz_add_affine(acc, pt);
     }
-    pt = next;
-    ent = ent_next;
+    ent0 = ent1;
+    ent1 = ent2;
+    ent2 = ent3;
+    pt0 = pt1;
+    pt1 = pt2;
   }
   if (cur != NONE)
     store_xyzz(slots + cur, acc);
