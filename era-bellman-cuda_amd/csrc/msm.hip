@@ -23,13 +23,16 @@
 //                  wave-ballot ranking, no global atomics, so heavy buckets
 //                  cost nothing extra), then run detection gives each bucket's
 //                  [start, end);
-//   3. accumulate: buckets are cut into pieces of at most `piece` points; one
-//                  thread accumulates one piece with XYZZ mixed additions, so a
-//                  heavy bucket (e.g. digit 1 of window 0 when many scalars are
-//                  1) is spread over many threads;
-//   4. fix-up:     buckets of up to 32 pieces are summed by one thread;
-//                  heavier buckets are listed and reduced by one-wave blocks
-//                  (chunks of 512 pieces: serial sums of 8, then an LDS tree);
+//   3. accumulate: the sorted entries of a job are cut into chunks of
+//                  `piece` entries, one per thread; a thread adds its chunk's
+//                  points with XYZZ mixed additions into one running sum per
+//                  bucket it meets, so every lane does the same work and a
+//                  heavy bucket (e.g. digit 1 of window 0 when many scalars
+//                  are 1) is spread over many threads;
+//   4. fix-up:     a bucket's sums lie in consecutive slots; buckets of more
+//                  than 32 are listed and reduced by one-wave blocks (chunks
+//                  of 512 sums: serial sums of 8, then an LDS tree); smaller
+//                  ones are summed as the buckets are gathered into windows;
 //   5. per-bit reduction: windows are split recursively into low/high halves
 //                  of their index bits until each holds one bit (sums split
 //                  across threads, then folded pairwise);
@@ -239,22 +242,25 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table 
 // instead leaves a wave waiting for its longest piece).
 
 // piece_offset[b], piece_count[b]: the slots of bucket b's sums, from its run
-// [start, end) of sorted entries (empty buckets have none)
+// [start, end) of sorted entries (empty buckets have none).  Buckets of more
+// than SMALL_PIECES sums (a few at most) are listed for the heavy fix-up.
 __global__ void bucket_slots_kernel(const job_table jobs, const uint32_t *start, const uint32_t *end, uint32_t *piece_offset, uint32_t *piece_count,
-                                    unsigned nb) {
+                                    unsigned nb, uint32_t *heavy, uint32_t *heavy_count) {
   const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb)
     return;
   const job_desc &jd = jobs.job[find_job(jobs, b, by_bucket())];
   const uint32_t s0 = start[b], e0 = end[b];
-  if (e0 <= s0) {
-    piece_count[b] = 0;
-    piece_offset[b] = 0;
-    return;
+  uint32_t np = 0, off = 0;
+  if (e0 > s0) {
+    const uint32_t k0 = (s0 - jd.entry_begin) / jd.piece, k1 = (e0 - 1 - jd.entry_begin) / jd.piece;
+    off = jd.slot_begin + (b - jd.bucket_begin) + k0;
+    np = k1 - k0 + 1;
   }
-  const uint32_t k0 = (s0 - jd.entry_begin) / jd.piece, k1 = (e0 - 1 - jd.entry_begin) / jd.piece;
-  piece_offset[b] = jd.slot_begin + (b - jd.bucket_begin) + k0;
-  piece_count[b] = k1 - k0 + 1;
+  piece_offset[b] = off;
+  piece_count[b] = np;
+  if (np > SMALL_PIECES)
+    heavy[atomicAdd(heavy_count, 1u)] = b;
 }
 
 __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t chunks,
@@ -307,27 +313,6 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     store_xyzz(slots + cur, acc);
 }
 
-// 4a. buckets of 2..SMALL_PIECES pieces are summed serially into piece 0;
-// larger ("heavy") buckets are listed for the block reduction below.
-__global__ __launch_bounds__(ACC_BLOCK) void small_merge_kernel(g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count, unsigned nb,
-                                                                uint32_t *heavy, uint32_t *heavy_count) {
-  const unsigned b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb)
-    return;
-  const uint32_t np = piece_count[b];
-  if (np < 2)
-    return;
-  if (np > SMALL_PIECES) {
-    heavy[atomicAdd(heavy_count, 1u)] = b;
-    return;
-  }
-  const uint32_t off = piece_offset[b];
-  g1_xyzz acc = load_xyzz(partials + off);
-  for (uint32_t q = 1; q < np; q++)
-    acc = xyzz_add(acc, load_xyzz(partials + off + q));
-  store_xyzz(partials + off, acc);
-}
-
 // Heavy buckets are reduced in chunks of HEAVY_CHUNK pieces, one block per
 // chunk: each thread sums HEAVY_PER_THREAD consecutive pieces, then a tree in
 // LDS; the chunk's sum overwrites the chunk's first piece.  A second kernel
@@ -362,7 +347,7 @@ __global__ __launch_bounds__(256) void heavy_prefix_kernel(const uint32_t *heavy
     *chunk_total = carry;
 }
 
-// 4b. one block per chunk (grid-stride over all chunks of all heavy buckets)
+// 4a. one block per chunk (grid-stride over all chunks of all heavy buckets)
 __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
                                                                   const uint32_t *chunk_offset, const uint32_t *chunk_total,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count) {
@@ -400,7 +385,7 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
   }
 }
 
-// 4c. one block per heavy bucket (grid-stride): its chunk sums, strided over
+// 4b. one block per heavy bucket (grid-stride): its chunk sums, strided over
 // the threads, then an LDS tree, into piece 0
 __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count) {
@@ -429,8 +414,9 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *parti
 // Level-0 windows: A[bucket_begin + w*2^(c-1) + k] for k in [0, 2^(c-1)) holds
 // bucket k of window w (k = 0 and the top digit hold infinity); tops[top_begin + w]
 // holds the bucket of the window's top digit (2^(c-1), or 2^top_signed in the last window).
-__global__ void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *partials, const uint32_t *piece_offset, const uint32_t *piece_count,
-                                     g1_xyzz *A, g1_xyzz *tops) {
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *partials,
+                                                                  const uint32_t *piece_offset, const uint32_t *piece_count, g1_xyzz *A,
+                                                                  g1_xyzz *tops) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nb)
     return;
@@ -439,22 +425,28 @@ __global__ void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1
   const uint32_t local = t - jd.bucket_begin, w = local >> (jd.c - 1), k = local & (B - 1);
   const uint32_t top_signed = jd.c - (jd.W * jd.c - 254) - 1;
   const uint32_t top_digit = w == jd.W - 1 ? (1u << top_signed) : B;
-  auto bucket_value = [&](uint32_t d) {
-    const uint32_t b = jd.bucket_begin + w * B + d - 1;
-    return piece_count[b] ? load_xyzz(partials + piece_offset[b]) : xyzz_infinity();
-  };
-  if (k == 0) {
+  // slot k of a window holds bucket k; slot 0 also hands the bucket of digit B
+  // to the window's top when that is its top digit, and the top digit's own
+  // slot in the last window hands it there instead
+  const bool to_top = k == 0 ? top_digit == B : k == top_digit;
+  if (k == 0 || k == top_digit)
     store_xyzz(A + t, xyzz_infinity());
-    if (top_digit == B)
-      store_xyzz(tops + jd.top_begin + w, bucket_value(B));
+  if (k == 0 && !to_top)
     return;
+  // bucket of digit d is batch bucket bucket_begin + w B + d - 1; buckets of
+  // 2..SMALL_PIECES sums are summed here, a heavy bucket's sum is already in
+  // its first slot
+  const uint32_t d = k == 0 ? B : k;
+  const uint32_t b = jd.bucket_begin + w * B + d - 1;
+  const uint32_t np = piece_count[b], off = piece_offset[b];
+  g1_xyzz acc = xyzz_infinity();
+  if (np != 0) {
+    acc = load_xyzz(partials + off);
+    if (np <= SMALL_PIECES)
+      for (uint32_t q = 1; q < np; q++)
+        acc = xyzz_add(acc, load_xyzz(partials + off + q));
   }
-  if (k == top_digit) {
-    store_xyzz(A + t, xyzz_infinity());
-    store_xyzz(tops + jd.top_begin + w, bucket_value(k));
-    return;
-  }
-  store_xyzz(A + t, bucket_value(k));
+  store_xyzz(to_top ? tops + jd.top_begin + w : A + t, acc);
 }
 
 // 5. one split level for every job whose windows are still wider than one bit.
@@ -689,36 +681,32 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   }
 
   // 3. accumulation over chunks of the sorted entries; meta = {heavy count, heavy chunk total}
-  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *meta = nullptr;
+  uint32_t *piece_count = nullptr, *piece_offset = nullptr, *meta = nullptr, *heavy = nullptr;
   g1_xyzz *partials = nullptr;
   BC_TRY(ar.get(&piece_count, nb));
   BC_TRY(ar.get(&piece_offset, nb));
   BC_TRY(ar.get(&meta, 2));
   BC_TRY(ar.get(&partials, slots));
+  BC_TRY(ar.get(&heavy, nb));
   BC_TRY(fill_u32(meta, 2, 0, s));
-  bucket_slots_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_offset, piece_count, nb);
+  bucket_slots_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_offset, piece_count, nb, heavy, meta);
   BC_LAUNCH_CHECK();
   accumulate_kernel<<<grid_for(chunks, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, (uint32_t)chunks, partials);
   BC_LAUNCH_CHECK();
 
-  // 4. fix-up of multi-piece buckets
-  {
-    uint32_t *heavy = nullptr, *chunk_offset = nullptr;
-    BC_TRY(ar.get(&heavy, nb));
+  // 4. buckets of more than SMALL_PIECES sums; smaller ones are summed by the gather
+  if (max_np > SMALL_PIECES) {
+    uint32_t *chunk_offset = nullptr;
     BC_TRY(ar.get(&chunk_offset, nb));
-    small_merge_kernel<<<grid_for(nb, ACC_BLOCK), ACC_BLOCK, 0, s>>>(partials, piece_offset, piece_count, nb, heavy, meta);
+    heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta, piece_count, chunk_offset, meta + 1);
     BC_LAUNCH_CHECK();
-    if (max_np > SMALL_PIECES) {
-      heavy_prefix_kernel<<<1, 256, 0, s>>>(heavy, meta, piece_count, chunk_offset, meta + 1);
-      BC_LAUNCH_CHECK();
-      heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, chunk_offset, meta + 1, piece_offset, piece_count);
-      BC_LAUNCH_CHECK();
-      heavy_final_kernel<<<256, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, piece_offset, piece_count);
-      BC_LAUNCH_CHECK();
-    }
-    BC_TRY(ar.put(heavy));
+    heavy_chunk_kernel<<<1024, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, chunk_offset, meta + 1, piece_offset, piece_count);
+    BC_LAUNCH_CHECK();
+    heavy_final_kernel<<<256, HEAVY_BLOCK, 0, s>>>(partials, heavy, meta, piece_offset, piece_count);
+    BC_LAUNCH_CHECK();
     BC_TRY(ar.put(chunk_offset));
   }
+  BC_TRY(ar.put(heavy));
 
   g1_xyzz *A = nullptr, *tops = nullptr;
   BC_TRY(ar.get(&A, nb));
