@@ -25,7 +25,8 @@ namespace {
 
 constexpr unsigned TUNE_MAX_LOG = 28;
 
-// Built-in window width and piece size of batched MSM jobs, per log size: the
+// Built-in window width and accumulation chunk (sorted entries per thread) of
+// batched MSM jobs, per log size: the
 // fastest per-job time of batches of equal jobs measured on MI355X (copies of
 // one job up to 2^22 points per batch; single jobs above).  Narrower windows
 // than the plain call's pay off in batches: fewer buckets to reduce per job.
@@ -176,8 +177,9 @@ hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const st
 
 // Temporaries of msm_batch_device for one job: 3 x 8 B per digit entry (the
 // (key, point) pairs, their sorted copy, the sort's ping-pong buffer), the
-// sort's tile histograms (at most 2 B per entry plus 8 KiB per window), 132 B
-// per piece (partial sum + bucket id), per bucket the run/piece tables and the
+// sort's tile histograms (at most 2 B per entry plus 8 KiB per window), a
+// 128 B partial-sum slot per bucket and per chunk of `piece` entries (132 B
+// counted), per bucket the run/slot tables, the heavy-bucket list and the
 // level-0 window array (152 B), and 128 B per bucket-split thread, with 256 B
 // of rounding per split level.
 size_t msm_footprint(const msm_job &j) {

@@ -11,7 +11,7 @@
 
 namespace bc {
 
-// Fills in the tuned window width / piece size; rejects malformed jobs.
+// Fills in the tuned window width / accumulation chunk; rejects malformed jobs.
 hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out);
 ntt_job resolve_ntt(const bc_amd_ntt_job &in);
 

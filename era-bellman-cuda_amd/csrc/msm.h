@@ -25,7 +25,7 @@ struct msm_job {
   g1_jacobian *results; // device, 254 points
   unsigned log_n;
   unsigned c;     // window bits, 3..24
-  unsigned piece; // most points one thread accumulates, >= 1
+  unsigned piece; // sorted (digit, point) entries per accumulating thread, 1..MSM_MAX_PIECE
 };
 
 // Runs up to MSM_MAX_JOBS MSMs with one launch per phase for the whole batch;

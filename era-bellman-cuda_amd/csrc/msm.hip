@@ -82,7 +82,7 @@ constexpr uint32_t SIGN = 0x80000000u;
 constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
 constexpr unsigned SPLIT_TERMS = 16; // most terms one split thread adds
-constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many pieces are summed by one thread
+constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many partial sums are summed by one thread
 
 struct job_desc {
   const g1_affine *bases;
@@ -313,10 +313,10 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     store_xyzz(slots + cur, acc);
 }
 
-// Heavy buckets are reduced in chunks of HEAVY_CHUNK pieces, one block per
-// chunk: each thread sums HEAVY_PER_THREAD consecutive pieces, then a tree in
-// LDS; the chunk's sum overwrites the chunk's first piece.  A second kernel
-// adds a bucket's chunk sums in order into its first piece.
+// Heavy buckets are reduced in chunks of HEAVY_CHUNK partial sums, one block
+// per chunk: each thread sums HEAVY_PER_THREAD consecutive sums, then a tree in
+// LDS; the chunk's sum overwrites the chunk's first slot.  A second kernel adds
+// a bucket's chunk sums in order into its first slot.
 constexpr unsigned HEAVY_BLOCK = 64;
 constexpr unsigned HEAVY_PER_THREAD = 8;
 constexpr unsigned HEAVY_CHUNK = HEAVY_BLOCK * HEAVY_PER_THREAD;
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
     const uint32_t b = heavy[lo];
     const uint32_t np = piece_count[b];
     g1_xyzz *base = partials + piece_offset[b] + (size_t)(c - chunk_offset[lo]) * HEAVY_CHUNK;
-    const uint32_t left = np - (c - chunk_offset[lo]) * HEAVY_CHUNK; // pieces from the chunk start
+    const uint32_t left = np - (c - chunk_offset[lo]) * HEAVY_CHUNK; // sums from the chunk start
     g1_xyzz acc = xyzz_infinity();
     for (uint32_t k = 0; k < HEAVY_PER_THREAD; k++) {
       const uint32_t q = threadIdx.x * HEAVY_PER_THREAD + k;
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
 }
 
 // 4b. one block per heavy bucket (grid-stride): its chunk sums, strided over
-// the threads, then an LDS tree, into piece 0
+// the threads, then an LDS tree, into the first slot
 __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count) {
   __shared__ g1_xyzz sh[HEAVY_BLOCK];

@@ -1,4 +1,4 @@
-// Process-wide tuning table: MSM window width and piece size per log size,
+// Process-wide tuning table: MSM window width and accumulation chunk per log size,
 // NTT first-pass stage count per log size.  Defaults are the values measured on
 // MI355X; the autotuner and bc_amd_tuning_load replace them.
 #pragma once
