@@ -175,11 +175,12 @@ __global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kernel(const ss_table t, 
 }
 
 // Key-value entries (key in the low word), ranked as above: one 8-byte load and
-// one scattered 8-byte store per entry instead of two of each.  On MI355X this
-// pass takes 1.75 ms for 218 M entries against 2.33 ms with separate key and
-// value arrays (the scattered stores, not their bytes, bound it).  Staging the
-// tile in LDS in digit order to coalesce the stores was slower (2.85 ms): its
-// 72 KiB of LDS leaves two blocks per CU.
+// one scattered 8-byte store per entry instead of two of each.  On MI355X a
+// pass over 218 M entries took 1.75 ms this way against 2.33 ms with separate
+// key and value arrays, and takes 1.3 ms since a thread's loads are all issued
+// before the first is ranked.  Staging the tile in LDS in digit order to
+// coalesce the stores was slower (2.85 ms: 72 KiB of LDS leaves two blocks per
+// CU), and so were 8192-entry tiles of 512 threads (fewer, longer barriers).
 __global__ __launch_bounds__(SS_BLOCK) void ss_scatter_kv_kernel(const ss_table t, const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                                  unsigned shift, unsigned bits, const uint32_t *__restrict__ table) {
   __shared__ uint32_t wave_hist[SS_WAVES][SS_MAX_DIGITS];
