@@ -137,9 +137,20 @@ template <bool GS> __global__ __launch_bounds__(NTT_BLOCK) void ntt_pass_kernel(
     return ((hi0 + g) << (L - s_lo)) | (t << log_lo) | (lo0 + c);
   };
   constexpr unsigned PER_THREAD = (1u << NTT_LOG_TILE) / NTT_BLOCK;
-  if (a.pre_table) {
-    // coset twist: one element at a time (the product's operands then never sit
-    // in a register tuple shared with other loads in flight)
+  if (a.pre_table && elems == (1u << NTT_LOG_TILE)) {
+    // coset twist: the elements and their twists are loaded first, then the
+    // products are formed
+    fe v[PER_THREAD], tw[PER_THREAD];
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++) {
+      const unsigned idx = index_of(threadIdx.x + r * NTT_BLOCK);
+      v[r] = load_fe(a.in + idx);
+      tw[r] = twist_pow(a.aux, GS ? brev_bits(idx, L) : idx);
+    }
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++)
+      tile[threadIdx.x + r * NTT_BLOCK] = mul_lazy<F>(v[r], tw[r]);
+  } else if (a.pre_table) {
     for (unsigned e = threadIdx.x; e < elems; e += NTT_BLOCK) {
       const unsigned idx = index_of(e);
       tile[e] = mul_lazy<F>(load_fe(a.in + idx), twist_pow(a.aux, GS ? brev_bits(idx, L) : idx));
