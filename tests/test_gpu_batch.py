@@ -108,7 +108,38 @@ def test_msm_batch_rejects_bad_jobs(bellman):
     p = dev.malloc(1024)
     bad = B.array_of(B.bc_amd_msm_job, [B.msm_job(p, p, p, 4, 25, 0)])
     assert B.lib().bc_amd_msm_batch_async(bad, ctypes.c_uint(1), dev.pool, dev.stream) == B.bc_error_invalid_value
+    bad = B.array_of(B.bc_amd_msm_job, [B.msm_job(p, p, p, 4, 8, 1 << 16)])  # chunk above 65535 entries
+    assert B.lib().bc_amd_msm_batch_async(bad, ctypes.c_uint(1), dev.pool, dev.stream) == B.bc_error_invalid_value
     dev.free(p)
+
+
+def test_msm_batch_chunk_edges(bellman, oracle, shared_bases):
+    """Chunk lengths at the edges of the accumulation's layout, batched together:
+    a chunk longer than the whole job (65535 entries), chunks of one entry, an
+    all-zero job (no bucket receives a point), an all-one job (window 0's
+    digit-1 bucket holds every point: a heavy bucket of 2^11 one-entry sums)
+    and r - 1 (every digit negated).  Each job's per-bit results equal its solo
+    run byte for byte, the oracle's projectively, and fold to the closed form."""
+    _, dev = bellman
+    db, bases, logs = shared_bases
+    R = oracle.R
+    cases = [(10, 8, 65535, None), (9, 5, 1, None), (8, 6, 16, [0] * 256), (11, 7, 1, [1] * 2048), (7, 9, 3, [R - 1] * 128)]
+    jobs, scal = [], []
+    for j, (L, c, piece, ints) in enumerate(cases):
+        k = oracle.to_mont(ints) if ints is not None else oracle.msm_scalars(0xBC0000A0 + j, 1 << L)
+        scal.append(k)
+        jobs.append(B.msm_job(db, dev.upload(k), dev.malloc(254 * 96), L, c, piece))
+    msm_batch(dev, jobs)
+    batched = [dev.download(job.results, (254, 24)) for job in jobs]
+    for j, (L, c, piece, _) in enumerate(cases):
+        msm_batch(dev, [jobs[j]])
+        assert np.array_equal(batched[j], dev.download(jobs[j].results, (254, 24))), (L, c, piece)
+        exp = oracle.msm(bases[: 1 << L], scal[j], L, c)
+        assert np.array_equal(oracle.to_affine(batched[j]), oracle.to_affine(exp)), (L, c, piece)
+        assert np.array_equal(oracle.to_affine(oracle.msm_fold(batched[j]))[0], closed_form(oracle, logs[: 1 << L], scal[j]))
+    for job in jobs:
+        dev.free(job.scalars)
+        dev.free(job.results)
 
 
 # (log n, bit-reversed in, inverse, log extension, coset)
