@@ -42,6 +42,24 @@ __global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of,
     start_of[k & key_mask] = p;
 }
 
+// Identity values omega_rows^row * k_col as hi_tab[col][row >> h] *
+// lo_tab[row mod 2^h], h = pn_split(log_rows): hi_tab holds
+// k_col * omega_rows^(hi 2^h), lo_tab omega_rows^lo (omega_rows^row =
+// omega_{2^28}^(row 2^(28 - log_rows))).
+__host__ __device__ constexpr unsigned pn_split(unsigned log_rows) { return (log_rows + 1) / 2; }
+
+__global__ void pn_tables_kernel(power_table w, const fe *scalars, fe *hi_tab, fe *lo_tab, unsigned cols, unsigned log_rows) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned h = pn_split(log_rows), n_lo = 1u << h, n_hi = 1u << (log_rows - h), shift = POW_LOG_RANGE - log_rows;
+  if (i < n_lo) {
+    store_fe(lo_tab + i, table_pow(w, i << shift));
+    return;
+  }
+  const unsigned j = i - n_lo; // j = col n_hi + hi
+  if (j < cols * n_hi)
+    store_fe(hi_tab + j, mul<FrParams>(table_pow(w, ((j % n_hi) << h) << shift), load_fe(scalars + j / n_hi)));
+}
+
 // In the reference's direction (src/pn_kernels.cu:89-121), sorted position j
 // of a run sends its value to the previous position, or to the run's last
 // position when j is the run's first; the first run maps to itself.  Here each
@@ -49,7 +67,7 @@ __global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of,
 // first position when p is the run's last (one table read per run), from
 // itself in the first run, and writes target[cell(p)] = omega^row * k_col of
 // that source cell.  A thread's loads are all independent but the last.
-__global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint64_t *sorted, const uint32_t *start_of, fe *target, unsigned n,
+__global__ void pn_generate_kernel(const fe *hi_tab, const fe *lo_tab, const uint64_t *sorted, const uint32_t *start_of, fe *target, unsigned n,
                                    unsigned log_rows, uint32_t key_mask) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n)
@@ -61,8 +79,9 @@ __global__ void pn_generate_kernel(power_table w, const fe *scalars, const uint6
     src = p + 1 < n && (uint32_t)next == k ? next : sorted[min(start_of[k & key_mask], n - 1)]; // clamped for indexes >= 2^bits
   const uint32_t src_cell = (uint32_t)(src >> 32), dst_cell = (uint32_t)(here >> 32);
   const uint32_t row = src_cell & ((1u << log_rows) - 1), col = src_cell >> log_rows;
-  const fe tw = table_pow(w, (row << (POW_LOG_RANGE - log_rows)) & ((1u << POW_LOG_RANGE) - 1));
-  store_fe(target + dst_cell, mul<FrParams>(tw, load_fe(scalars + col)));
+  const unsigned h = pn_split(log_rows);
+  const fe hi = load_fe(hi_tab + ((size_t)col << (log_rows - h)) + (row >> h)), lo = load_fe(lo_tab + (row & ((1u << h) - 1)));
+  store_fe(target + dst_cell, mul<FrParams>(hi, lo));
 }
 
 } // namespace
@@ -91,8 +110,16 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   BC_TRY(ar.get(&start_of, (size_t)key_mask + 1));
   pn_run_starts_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, start_of, n, key_mask);
   BC_LAUNCH_CHECK();
-  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, sorted, start_of, target, n, L, key_mask);
+  fe *hi_tab = nullptr, *lo_tab = nullptr;
+  const unsigned h = pn_split(L);
+  BC_TRY(ar.get(&hi_tab, (size_t)cols << (L - h)));
+  BC_TRY(ar.get(&lo_tab, (size_t)1 << h));
+  pn_tables_kernel<<<grid_for((1u << h) + (cols << (L - h)), PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, hi_tab, lo_tab, cols, L);
   BC_LAUNCH_CHECK();
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(hi_tab, lo_tab, sorted, start_of, target, n, L, key_mask);
+  BC_LAUNCH_CHECK();
+  BC_TRY(ar.put(lo_tab));
+  BC_TRY(ar.put(hi_tab));
   BC_TRY(ar.put(start_of));
   BC_TRY(ar.put(sorted));
   return hipSuccess;
