@@ -176,8 +176,10 @@ hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const st
 }
 
 // Temporaries of msm_batch_device for one job: 3 x 8 B per digit entry (the
-// (key, point) pairs, their sorted copy, the sort's ping-pong buffer), the
-// sort's tile histograms (at most 2 B per entry plus 8 KiB per window), a
+// (key, point) pairs, their sorted copy, the sort's ping-pong buffer), two
+// tile-histogram tables (the first pass's, counted by the digits kernel, and
+// the sort's own for later passes: each at most 2 B per entry plus 8 KiB per
+// window), a
 // 128 B partial-sum slot per bucket and per chunk of `piece` entries (132 B
 // counted), per bucket the run/slot tables, the heavy-bucket list and the
 // level-0 window array (152 B), and 128 B per bucket-split thread, with 256 B
@@ -185,7 +187,7 @@ hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const st
 size_t msm_footprint(const msm_job &j) {
   const uint64_t W = (254 - 1) / j.c + 1;
   const uint64_t E = W << j.log_n, nb = W << (j.c - 1);
-  return (size_t)(26 * E + 8192 * W + 132 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c) + 256 * 8 + (1u << 20));
+  return (size_t)(28 * E + 16384 * W + 132 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c) + 256 * 8 + (1u << 20));
 }
 
 // Coset transforms hold one twist table (2^12 fine + 2^(L-12) coarse entries).

@@ -83,21 +83,25 @@ constexpr unsigned MSM_BLOCK = 256;
 constexpr unsigned ACC_BLOCK = 64;
 constexpr unsigned SPLIT_TERMS = 16; // most terms one split thread adds
 constexpr unsigned SMALL_PIECES = 32; // buckets of up to this many partial sums are summed by one thread
+// First-digit counters of one digits block.  Measured at 2^24: with 15 KiB of
+// counters (c = 17) counting in the digits kernel saved 0.18 ms over the
+// sort's own count; with 27 KiB (c = 20) it saved nothing.
+constexpr size_t DIGITS_MAX_LDS = 20 * 1024;
 
 struct job_desc {
   const g1_affine *bases;
   const fe *scalars;
   uint32_t entry_begin;  // first (key, value) pair of the job (a launch holds < 2^31)
-  uint32_t n;            // scalars
-  uint32_t scalar_begin;
+  uint32_t digit_block;  // first block of the job in the digits launch
+  uint32_t table_begin;  // the job's counts in the sort's first-pass table
   uint32_t bucket_begin; // first batch-global bucket
   uint32_t top_begin;    // first top-digit slot
   uint32_t chunk_begin;  // first accumulation chunk of the job
   uint32_t slot_begin;   // first partial-sum slot of the job
   uint16_t piece;        // sorted entries per accumulation chunk
-  uint8_t log_n;
-  uint8_t c; // window bits
-  uint8_t W; // windows
+  uint8_t log_n;         // 2^log_n scalars
+  uint8_t c;             // window bits
+  uint8_t W;             // windows
 };
 
 static_assert(MSM_MAX_JOBS <= SEG_MAX_GROUPS, "one sort segment group per job");
@@ -121,8 +125,8 @@ template <class Begin> __device__ __forceinline__ unsigned find_job(const job_ta
   return lo;
 }
 
-struct by_scalar {
-  __device__ uint32_t operator()(const job_desc &d) const { return d.scalar_begin; }
+struct by_digit_block {
+  __device__ uint32_t operator()(const job_desc &d) const { return d.digit_block; }
 };
 struct by_bucket {
   __device__ uint32_t operator()(const job_desc &d) const { return d.bucket_begin; }
@@ -148,32 +152,81 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 }
 
 // 1. signed digits.  Entry (w, i) of a job gets the window-local key d - 1
-// (2^(c-1), after every bucket, for d = 0) and value = sign << 31 | i.
-__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint32_t total, uint64_t *__restrict__ entries) {
+// (2^(c-1), after every bucket, for d = 0) and value = sign << 31 | i.  Block k
+// of a job takes points [k block_points, (k+1) block_points).  With
+// `first_table`, block_points = SEG_TILE, so a block holds tile k of every
+// window's sort segment: it also counts the sort's first digit (key bits
+// [0, bits0)) per window in LDS and writes the tile's counts to the first-pass
+// table, and the sort need not read the entries to count them.  Without it a
+// block takes MSM_BLOCK points, one per thread.  A scalar is loaded one point
+// ahead of its digits.
+__global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint64_t *__restrict__ entries, uint32_t *__restrict__ first_table,
+                                                           unsigned bits0, uint32_t block_points) {
   using F = FrParams;
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total)
-    return;
-  const job_desc &jd = jobs.job[find_job(jobs, t, by_scalar())];
-  const uint32_t i = t - jd.scalar_begin, n = jd.n, c = jd.c;
-  const fe pos = from_mont<F>(load_fe(jd.scalars + i));
-  fe neg;
-  sub_limbs(neg, fe_const<F>(F::m), pos); // r - s (r for s = 0)
-  const uint32_t global_sign = lt(pos, neg) ? 0u : SIGN;
-  const fe k = global_sign ? neg : pos;
+  // [W][2^min(c, bits0)] 16-bit counts (a tile has SEG_TILE entries), two per
+  // word: digit d of window w is half d & 1 of word (w nd + d) / 2
+  extern __shared__ uint32_t hist[];
+  // blocks are dealt round-robin to the 8 XCDs; block ids are remapped so each
+  // XCD takes a contiguous range of tiles, and the counts that neighbouring
+  // tiles write side by side into the table meet in one L2
+  const uint32_t per_xcd = gridDim.x >> 3;
+  const uint32_t blk = blockIdx.x < (per_xcd << 3) ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  const job_desc &jd = jobs.job[find_job(jobs, blk, by_digit_block())];
+  const uint32_t k = blk - jd.digit_block, n = 1u << jd.log_n, c = jd.c, W = jd.W;
   const uint32_t top = 1u << (c - 1);
-  uint32_t borrow = 0;
-  for (unsigned w = 0; w < jd.W; w++) {
-    uint32_t d = extract_bits(k, w * c, c) + borrow;
-    uint32_t sign = global_sign;
-    borrow = 0;
-    if (d > top) {
-      d = (top << 1) - d;
-      borrow = 1;
-      sign ^= SIGN;
+  const unsigned dbits = c < bits0 ? c : bits0; // keys are below 2^c
+  const uint32_t nd = 1u << dbits;
+  auto count = [&](uint32_t w, uint32_t d, uint32_t v) {
+    const uint32_t slot = w * nd + d;
+    atomicAdd(&hist[slot >> 1], v << ((slot & 1) << 4));
+  };
+  if (first_table) {
+    for (uint32_t i = threadIdx.x; i < (W * nd + 1) / 2; i += MSM_BLOCK)
+      hist[i] = 0;
+    __syncthreads();
+  }
+  const uint32_t p1 = min(n, (k + 1) * block_points);
+  uint32_t i = k * block_points + threadIdx.x;
+  fe next = i < p1 ? load_fe(jd.scalars + i) : fe_zero();
+  for (; i < p1; i += MSM_BLOCK) {
+    const fe pos = from_mont<F>(next);
+    if (i + MSM_BLOCK < p1)
+      next = load_fe(jd.scalars + i + MSM_BLOCK);
+    fe neg;
+    sub_limbs(neg, fe_const<F>(F::m), pos); // r - s (r for s = 0)
+    const uint32_t global_sign = lt(pos, neg) ? 0u : SIGN;
+    const fe kk = global_sign ? neg : pos;
+    uint32_t borrow = 0;
+    for (unsigned w = 0; w < W; w++) {
+      uint32_t d = extract_bits(kk, w * c, c) + borrow;
+      uint32_t sign = global_sign;
+      borrow = 0;
+      if (d > top) {
+        d = (top << 1) - d;
+        borrow = 1;
+        sign ^= SIGN;
+      }
+      const uint32_t key = d ? d - 1 : top;
+      entries[jd.entry_begin + ((uint64_t)w << jd.log_n) + i] = (uint64_t)(sign | i) << 32 | key; // key low, point reference high
+      if (first_table) {
+        // zero digits (often most lanes of a wave in the high windows) are
+        // counted once per wave instead of once per lane
+        const uint64_t zeros = __ballot(d == 0);
+        if (d != 0)
+          count(w, key & (nd - 1), 1u);
+        else if ((threadIdx.x & 63) == (unsigned)__ffsll((unsigned long long)zeros) - 1)
+          count(w, top & (nd - 1), (uint32_t)__popcll(zeros));
+      }
     }
-    const uint64_t e = jd.entry_begin + (uint64_t)w * n + i;
-    entries[e] = (uint64_t)(sign | i) << 32 | (d ? d - 1 : top); // key low, point reference high
+  }
+  if (first_table) {
+    __syncthreads();
+    // window w, digit d of tile k; digits at or above 2^c count nothing
+    const uint32_t tiles = (n + SEG_TILE - 1) / SEG_TILE;
+    for (uint32_t j = threadIdx.x; j < (W << bits0); j += MSM_BLOCK) {
+      const uint32_t w = j >> bits0, d = j & ((1u << bits0) - 1), slot = w * nd + d;
+      first_table[jd.table_begin + (uint64_t)j * tiles + k] = d < nd ? (hist[slot >> 1] >> ((slot & 1) << 4)) & 0xffffu : 0u;
+    }
   }
 }
 
@@ -187,7 +240,7 @@ __device__ __forceinline__ void bucket_run_ends(const job_desc &jd, uint32_t e, 
   if (k == top)
     return; // zero digits
   const uint32_t local = e - jd.entry_begin, w = local >> jd.log_n;
-  const uint32_t seg_begin = jd.entry_begin + w * jd.n, seg_end = seg_begin + jd.n;
+  const uint32_t seg_begin = jd.entry_begin + (w << jd.log_n), seg_end = seg_begin + (1u << jd.log_n);
   const uint32_t b = jd.bucket_begin + w * top + k;
   if (e == seg_begin || k_prev != k)
     start[b] = e;
@@ -620,36 +673,34 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   job_table jt{};
   jt.count = count;
   uint64_t entries = 0, nb64 = 0, chunks = 0, slots = 0;
-  uint32_t scalars_total = 0, tops_total = 0, max_np = 0;
+  uint32_t tops_total = 0, max_np = 0;
   for (unsigned j = 0; j < count; j++) {
     const unsigned c = jobs[j].c;
     if (c < 3 || c > 24 || jobs[j].log_n > 28 || jobs[j].piece == 0 || jobs[j].piece > MSM_MAX_PIECE || !jobs[j].bases || !jobs[j].scalars ||
         !jobs[j].results)
       return hipErrorInvalidValue;
     job_desc &d = jt.job[j];
+    const uint32_t n = 1u << jobs[j].log_n;
     d.bases = jobs[j].bases;
     d.scalars = jobs[j].scalars;
-    d.n = 1u << jobs[j].log_n;
     d.log_n = (uint8_t)jobs[j].log_n;
     d.c = (uint8_t)c;
     d.W = (uint8_t)((254 - 1) / c + 1);
     d.piece = (uint16_t)jobs[j].piece;
-    if ((uint64_t)scalars_total + d.n > 0xffffffffull || entries + (uint64_t)d.W * d.n >= (1ull << 31))
+    if (entries + (uint64_t)d.W * n >= (1ull << 31))
       return hipErrorInvalidValue;
-    d.scalar_begin = scalars_total;
     d.entry_begin = (uint32_t)entries;
     d.bucket_begin = (uint32_t)nb64;
     d.top_begin = tops_total;
     d.chunk_begin = (uint32_t)chunks;
     d.slot_begin = (uint32_t)slots;
-    const uint64_t job_chunks = ((uint64_t)d.W * d.n + d.piece - 1) / d.piece, job_buckets = (uint64_t)d.W << (c - 1);
-    scalars_total += d.n;
-    entries += (uint64_t)d.W * d.n;
+    const uint64_t job_chunks = ((uint64_t)d.W * n + d.piece - 1) / d.piece, job_buckets = (uint64_t)d.W << (c - 1);
+    entries += (uint64_t)d.W * n;
     nb64 += job_buckets;
     tops_total += d.W;
     chunks += job_chunks;
     slots += job_buckets + job_chunks; // slot(b, k) <= (buckets - 1) + (chunks - 1)
-    max_np = std::max<uint32_t>(max_np, (d.n + d.piece - 1) / d.piece + 1); // a bucket holds at most n entries
+    max_np = std::max<uint32_t>(max_np, (n + d.piece - 1) / d.piece + 1); // a bucket holds at most n entries
   }
   if (entries >= (1ull << 31) || nb64 >= (1ull << 31) || slots >= (1ull << 32))
     return hipErrorInvalidValue;
@@ -664,15 +715,36 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   {
     uint64_t *digits = nullptr;
     BC_TRY(ar.get(&digits, entries));
-    digits_kernel<<<grid_for(scalars_total, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, scalars_total, digits);
-    BC_LAUNCH_CHECK();
     seg_group groups[MSM_MAX_JOBS];
     unsigned key_bits = 0; // local keys are <= 2^(c-1)
     for (unsigned j = 0; j < count; j++) {
-      groups[j] = {jt.job[j].entry_begin, jt.job[j].n, jt.job[j].W};
+      groups[j] = {jt.job[j].entry_begin, 1u << jt.job[j].log_n, jt.job[j].W};
       key_bits = std::max<unsigned>(key_bits, jt.job[j].c);
     }
-    BC_TRY(segmented_sort_kv(digits, sorted, groups, count, key_bits, ar));
+    // the digits kernel counts the sort's first digit when its per-window
+    // counters fit DIGITS_MAX_LDS; otherwise the sort counts them itself
+    seg_first_pass fp;
+    BC_TRY(segmented_sort_first_pass(groups, count, key_bits, &fp));
+    size_t lds = 0; // 16-bit counters, two per word
+    for (unsigned j = 0; j < count; j++)
+      lds = std::max(lds, sizeof(uint32_t) * ((((size_t)jt.job[j].W << std::min<unsigned>(jt.job[j].c, fp.bits)) + 1) / 2));
+    uint32_t *first_table = nullptr;
+    if (fp.bits > 0 && lds <= DIGITS_MAX_LDS && fp.table_entries < (1ull << 32)) {
+      BC_TRY(ar.get(&first_table, fp.table_entries));
+      for (unsigned j = 0; j < count; j++)
+        jt.job[j].table_begin = (uint32_t)fp.table_begin[j];
+    }
+    const uint32_t block_points = first_table ? SEG_TILE : MSM_BLOCK;
+    uint32_t blocks = 0;
+    for (unsigned j = 0; j < count; j++) {
+      jt.job[j].digit_block = blocks;
+      blocks += ((1u << jt.job[j].log_n) + block_points - 1) / block_points;
+    }
+    digits_kernel<<<blocks, MSM_BLOCK, first_table ? lds : 0, s>>>(jt, digits, first_table, fp.bits, block_points);
+    BC_LAUNCH_CHECK();
+    BC_TRY(segmented_sort_kv(digits, sorted, groups, count, key_bits, ar, first_table));
+    if (first_table)
+      BC_TRY(ar.put(first_table));
     BC_TRY(ar.put(digits));
     BC_TRY(fill_u32(offsets, nb, 0, s));
     BC_TRY(fill_u32(counts, nb, 0, s));

@@ -19,6 +19,7 @@ constexpr unsigned SS_BLOCK = 256;
 constexpr unsigned SS_WAVES = SS_BLOCK / 64;
 constexpr unsigned SS_ITEMS = 16; // entries per thread
 constexpr unsigned SS_TILE = SS_BLOCK * SS_ITEMS;
+static_assert(SS_TILE == SEG_TILE, "tile size shared with first-pass producers");
 constexpr unsigned SS_MAX_BITS = 11;
 constexpr unsigned SS_MAX_DIGITS = 1u << SS_MAX_BITS;
 
@@ -297,7 +298,23 @@ hipError_t ss_make_plan(const seg_group *groups, unsigned group_count, unsigned 
 
 } // namespace
 
-hipError_t segmented_sort_kv(const uint64_t *in, uint64_t *out, const seg_group *groups, unsigned group_count, unsigned key_bits, arena &ar) {
+hipError_t segmented_sort_first_pass(const seg_group *groups, unsigned group_count, unsigned key_bits, seg_first_pass *out) {
+  ss_plan pl;
+  BC_TRY(ss_make_plan(groups, group_count, key_bits, pl));
+  *out = seg_first_pass{};
+  if (pl.passes == 0)
+    return hipSuccess;
+  out->bits = pl.widths[0];
+  out->table_entries = pl.slots[0];
+  for (unsigned g = 0; g < group_count; g++) {
+    out->table_begin[g] = pl.t[0].g[g].table_begin;
+    out->tiles_per_seg[g] = pl.t[0].g[g].tiles_per_seg;
+  }
+  return hipSuccess;
+}
+
+hipError_t segmented_sort_kv(const uint64_t *in, uint64_t *out, const seg_group *groups, unsigned group_count, unsigned key_bits, arena &ar,
+                             uint32_t *first_table) {
   hipStream_t stream = ar.stream();
   if (group_count == 0)
     return hipSuccess;
@@ -307,22 +324,29 @@ hipError_t segmented_sort_kv(const uint64_t *in, uint64_t *out, const seg_group 
     return hipSuccess;
   if (pl.passes == 0)
     return hipMemcpyAsync(out, in, pl.n * 8, hipMemcpyDeviceToDevice, stream);
+  // with the first pass counted by the producer, the own table serves the
+  // later passes only (pass widths do not grow)
   uint32_t *table = nullptr;
   uint64_t *tmp = nullptr;
-  BC_TRY(ar.get(&table, pl.table_entries));
+  const uint64_t own = first_table ? (pl.passes > 1 ? pl.slots[1] : 0) : pl.table_entries;
+  if (own)
+    BC_TRY(ar.get(&table, own));
   BC_TRY(ar.get(&tmp, pl.n));
   const uint64_t *src = in;
   for (unsigned p = 0; p < pl.passes; p++) {
     uint64_t *dst = ((pl.passes - 1 - p) % 2) == 0 ? out : tmp; // the last pass lands in `out`
-    ss_histogram_kernel<uint64_t><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, pl.shifts[p], pl.widths[p], table);
-    BC_LAUNCH_CHECK();
-    BC_TRY(scan_u32_exclusive(table, table, pl.slots[p], ar));
-    ss_scatter_kv_kernel<<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, dst, pl.shifts[p], pl.widths[p], table);
+    uint32_t *tab = p == 0 && first_table ? first_table : table;
+    if (tab != first_table) {
+      ss_histogram_kernel<uint64_t><<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, pl.shifts[p], pl.widths[p], tab);
+      BC_LAUNCH_CHECK();
+    }
+    BC_TRY(scan_u32_exclusive(tab, tab, pl.slots[p], ar));
+    ss_scatter_kv_kernel<<<pl.t[p].tiles, SS_BLOCK, 0, stream>>>(pl.t[p], src, dst, pl.shifts[p], pl.widths[p], tab);
     BC_LAUNCH_CHECK();
     src = dst;
   }
   BC_TRY(ar.put(tmp));
-  return ar.put(table);
+  return table ? ar.put(table) : hipSuccess;
 }
 
 hipError_t segmented_sort_pairs(const uint32_t *keys_in, uint32_t *keys_out, const uint32_t *values_in, uint32_t *values_out, const seg_group *groups,
