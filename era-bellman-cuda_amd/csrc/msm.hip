@@ -555,14 +555,26 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
     x = L >> (log_d + t);
   }
   const g1_xyzz *src = sg.A + ((size_t)x << s);
+  // terms q = first, first + step, ... < limit at src[(q << t) | j] (low half)
+  // or src[(j << t) | q] (high half); the next term is loaded before the
+  // current one is added
+  const bool lo = half == 0;
+  const uint32_t first = part, step = lo ? sg.D_lo : sg.D, limit = lo ? (1u << hs) : (1u << t);
+  const bool active = lo ? part < sg.D_lo : j < (1u << hs);
+  auto term = [&](uint32_t q) { return src + (lo ? ((q << t) | j) : ((j << t) | q)); };
   g1_xyzz acc = xyzz_infinity();
-  if (half == 0) {
-    if (part < sg.D_lo)
-      for (uint32_t v = part; v < (1u << hs); v += sg.D_lo)
-        acc = xyzz_add(acc, load_xyzz(src + ((v << t) | j)));
-  } else if (j < (1u << hs)) {
-    for (uint32_t u = part; u < (1u << t); u += sg.D)
-      acc = xyzz_add(acc, load_xyzz(src + ((j << t) | u)));
+  if (active && first < limit) {
+    acc = load_xyzz(term(first));
+    uint32_t q = first + step;
+    if (q < limit) {
+      g1_xyzz cur = load_xyzz(term(q));
+      for (q += step; q < limit; q += step) {
+        const g1_xyzz next = load_xyzz(term(q));
+        acc = xyzz_add(acc, cur);
+        cur = next;
+      }
+      acc = xyzz_add(acc, cur);
+    }
   }
   store_xyzz(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, acc);
 }
