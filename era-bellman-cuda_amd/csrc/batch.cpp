@@ -32,7 +32,7 @@ constexpr unsigned TUNE_MAX_LOG = 28;
 // than the plain call's pay off in batches: fewer buckets to reduce per job.
 msm_tuning msm_builtin_tuning(unsigned l) {
   static const msm_tuning measured[] = {
-      {5, 32},  /* 12 */ {9, 8},   /* 13 */ {7, 32},  /* 14 */ {11, 16}, /* 15 */ {11, 8},   /* 16 */
+      {9, 32},  /* 12 */ {9, 8},   /* 13 */ {11, 16}, /* 14 */ {11, 16}, /* 15 */ {11, 8},   /* 16 */
       {13, 16}, /* 17 */ {13, 32}, /* 18 */ {15, 32}, /* 19 */ {15, 16}, /* 20 */ {17, 32},  /* 21 */
       {17, 32}, /* 22 */ {17, 64}, /* 23 */ {17, 128}, /* 24 */
   };

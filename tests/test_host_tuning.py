@@ -28,7 +28,7 @@ def test_msm_tuning_builtin_table(B):
     pkg = os.path.dirname(os.path.dirname(B.__file__))
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True, env=dict(os.environ, PYTHONPATH=pkg))
     table = {int(a): (int(b), int(c)) for a, b, c in (line.split() for line in out.stdout.splitlines())}
-    assert table[12] == (5, 32) and table[16] == (11, 8) and table[20] == (15, 16) and table[24] == (17, 128)
+    assert table[12] == (9, 32) and table[14] == (11, 16) and table[16] == (11, 8) and table[20] == (15, 16) and table[24] == (17, 128)
     assert table[10] == (5, 16) and table[9] == (4, 16) and table[0] == (3, 16)
     assert table[25] == (20, 128) and table[26] == (22, 128) and table[28] == (22, 128)
     for L, (c, piece) in table.items():
