@@ -9,9 +9,12 @@ namespace bc {
 
 // Sorted (digit, point) entries per accumulating thread of the plain call at
 // the reference window widths, measured on MI355X: short chunks keep small
-// MSMs parallel, long ones cut the partial sums of large MSMs (2^24: 64 and
-// 128 tie, 8 is 11% slower).
-constexpr unsigned msm_plain_piece(unsigned log_n) { return log_n >= 24 ? 64u : log_n >= 21 ? 32u : log_n >= 19 ? 16u : 8u; }
+// MSMs parallel (up to 2^15 the call is latency-bound and 4 entries per thread
+// was fastest in two sweeps of three sizes), long ones cut the partial sums of
+// large MSMs (2^24: 128 entries 2 % faster than 64, 8 % faster than 16).
+constexpr unsigned msm_plain_piece(unsigned log_n) {
+  return log_n >= 24 ? 128u : log_n == 23 ? 64u : log_n >= 21 ? 32u : log_n >= 19 ? 16u : log_n >= 16 ? 8u : 4u;
+}
 // The most entries per accumulating thread a job may ask for.
 constexpr unsigned MSM_MAX_PIECE = 0xffff;
 constexpr unsigned MSM_MAX_JOBS = 64;
