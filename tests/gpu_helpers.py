@@ -62,6 +62,46 @@ def msm(dev, d_bases, scalars, log_n, host_scalars=False, host_results=False):
     return res
 
 
+class CopyHooks:
+    """The h2d/d2h events and callbacks of msm_configuration and
+    ntt_configuration: two events, and one host function that records the
+    user data it is called with (0x11 for h2d, 0x22 for d2h)."""
+
+    H2D, D2H = 0x11, 0x22
+
+    def __init__(self):
+        self.calls = []
+        self.events = []
+        for _ in range(2):
+            ev = B.bc_event()
+            B.call("bc_event_create", ctypes.byref(ev), ctypes.c_bool(False), ctypes.c_bool(True))
+            self.events.append(ev)
+        self.fn = B.bc_host_fn(lambda data: self.calls.append(data))
+
+    def attach(self, cfg):
+        cfg.h2d_copy_finished, cfg.d2h_copy_finished = self.events
+        cfg.h2d_copy_finished_callback = cfg.d2h_copy_finished_callback = self.fn
+        cfg.h2d_copy_finished_callback_data = self.H2D
+        cfg.d2h_copy_finished_callback_data = self.D2H
+
+    def wait_calls(self, count, seconds=5.0):
+        """The host functions run on a runtime thread; a stream synchronize
+        returns once they have, but allow a bounded grace period."""
+        import time
+
+        end = time.time() + seconds
+        while len(self.calls) < count and time.time() < end:
+            time.sleep(0.01)
+        return list(self.calls)
+
+    def events_complete(self):
+        return all(B.lib().bc_event_query(ev) == 0 for ev in self.events)
+
+    def close(self):
+        for ev in self.events:
+            B.call("bc_event_destroy", ev)
+
+
 def ff_binary(dev, name, a, x, y, a_on_device=False):
     """Element-wise ops; `a` passed by host pointer or device pointer."""
     x = np.ascontiguousarray(x, dtype=np.uint32)

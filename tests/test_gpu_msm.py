@@ -5,7 +5,7 @@ Jacobian limbs are not canonical), and their fold against the closed form
 import numpy as np
 import pytest
 
-from gpu_helpers import msm
+from gpu_helpers import CopyHooks, msm
 
 pytestmark = pytest.mark.gpu
 
@@ -132,3 +132,42 @@ def test_msms_overlap_on_four_streams(bellman, oracle):
             B.call("bc_stream_destroy", st)
         for p in d_s + d_r + [db]:
             dev.free(p)
+
+
+@pytest.mark.parametrize("host_scalars,host_results", [(True, True), (True, False), (False, True), (False, False)])
+def test_msm_copy_hooks(bellman, oracle, host_scalars, host_results):
+    """The h2d/d2h events and callbacks of msm_configuration
+    (src/msm.cu:183-191,394-400): the h2d callback runs once after host
+    scalars are copied in, the d2h callback once after host results are copied
+    out, in that order and with their user data, and the events complete; a
+    buffer already on the device is not copied and its callback does not run.
+    The results equal those of the device-buffer call."""
+    B, dev = bellman
+    L = 9
+    s = oracle.from_limbs(oracle.random_fr(0xBC000071, 1 << L, montgomery=False))
+    db = dev.upload(oracle.gen_bases(s))
+    k = oracle.msm_scalars(0xBC000072, 1 << L)
+    want = msm(dev, db, k, L)
+    hooks = CopyHooks()
+    res_host = np.zeros((254, 24), dtype=np.uint32)
+    d_s = None if host_scalars else dev.upload(k)
+    d_r = None if host_results else dev.malloc(254 * 96)
+    try:
+        cfg = B.msm_configuration()
+        cfg.mem_pool, cfg.stream, cfg.bases = dev.pool, dev.stream, db
+        cfg.scalars = k.ctypes.data if host_scalars else d_s
+        cfg.results = res_host.ctypes.data if host_results else d_r
+        cfg.log_scalars_count = L
+        hooks.attach(cfg)
+        B.call("msm_execute_async", cfg)
+        dev.sync()
+        got = res_host if host_results else dev.download(d_r, (254, 24))
+        assert np.array_equal(got, want)
+        expected = [CopyHooks.H2D] * host_scalars + [CopyHooks.D2H] * host_results
+        assert hooks.wait_calls(len(expected)) == expected
+        assert hooks.events_complete()
+    finally:
+        hooks.close()
+        for p in (d_s, d_r, db):
+            if p is not None:
+                dev.free(p)

@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from gpu_helpers import ntt
+from gpu_helpers import CopyHooks, ntt
 
 pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -92,3 +92,39 @@ def test_lde_assembled_in_whole_domain_bit_reversed_order(bellman, oracle):
     y = np.zeros(((1 << e) * n, 8), dtype=np.uint32)
     y[:n] = oracle.multiply_by_powers(x, oracle.to_mont([7]))
     assert np.array_equal(got, oracle.ntt(y, L + e))
+
+
+@pytest.mark.parametrize("host_in,host_out", [(True, True), (True, False), (False, True), (False, False)])
+def test_ntt_copy_hooks(bellman, oracle, host_in, host_out):
+    """The h2d/d2h events and callbacks of ntt_configuration
+    (src/ntt.cu:170-180,220-229): the h2d callback runs once after host inputs
+    are copied in, the d2h callback once after host outputs are copied out, in
+    that order and with their user data, and the events complete; a buffer
+    already on the device is not copied and its callback does not run.  The
+    outputs are the oracle's."""
+    B, dev = bellman
+    L = 10
+    x = oracle.random_fr(0xBC000073, 1 << L)
+    hooks = CopyHooks()
+    out_host = np.zeros((1 << L, 8), dtype=np.uint32)
+    d_in = None if host_in else dev.upload(x)
+    d_out = None if host_out else dev.malloc(32 << L)
+    try:
+        cfg = B.ntt_configuration()
+        cfg.mem_pool, cfg.stream = dev.pool, dev.stream
+        cfg.inputs = x.ctypes.data if host_in else d_in
+        cfg.outputs = out_host.ctypes.data if host_out else d_out
+        cfg.log_values_count = L
+        hooks.attach(cfg)
+        B.call("ntt_execute_async", cfg)
+        dev.sync()
+        got = out_host if host_out else dev.download(d_out, (1 << L, 8))
+        assert np.array_equal(got, oracle.ntt(x, L, False, False))
+        expected = [CopyHooks.H2D] * host_in + [CopyHooks.D2H] * host_out
+        assert hooks.wait_calls(len(expected)) == expected
+        assert hooks.events_complete()
+    finally:
+        hooks.close()
+        for p in (d_in, d_out):
+            if p is not None:
+                dev.free(p)
