@@ -262,6 +262,63 @@ def test_pn_sparse_variable_indexes(bellman, oracle, cols, log_rows):
     assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), oracle.pn(idx, ks, cols, log_rows))
 
 
+@pytest.mark.parametrize("cols,log_rows,pattern", [(4, 6, "one_variable"), (3, 7, "one_variable_nonzero"), (4, 6, "all_distinct"),
+                                                    (3, 5, "all_distinct"), (4, 0, "one_variable"), (3, 0, "random"), (4, 1, "random"),
+                                                    (3, 2, "random")])
+def test_pn_degenerate_indexes(bellman, oracle, cols, log_rows, pattern):
+    """Edge cases of the index matrix: one variable in every cell (a single run,
+    the first, which maps to itself), every cell its own variable (runs of one
+    cell), and matrices of one, two and four rows.  The first two are the
+    identity permutation, so their values are omega^row * k_col whatever the
+    variable numbering: both must equal the oracle and each other."""
+    _, dev = bellman
+    cells = cols << log_rows
+    rng = random.Random(31 * cells + log_rows)
+    if pattern == "one_variable":
+        idx = np.zeros(cells, dtype=np.uint32)
+    elif pattern == "one_variable_nonzero":
+        idx = np.full(cells, 5, dtype=np.uint32)
+    elif pattern == "all_distinct":
+        idx = np.array(rng.sample(range(cells), cells), dtype=np.uint32)
+    else:
+        perm = list(range(cells))
+        rng.shuffle(perm)
+        idx = np.zeros(cells, dtype=np.uint32)
+        pos = var = 0
+        while pos < cells:
+            L = rng.randint(1, 4)
+            idx[perm[pos:pos + L]] = var
+            pos += L
+            var += 1
+    ks = oracle.random_fr(cells + 2, cols)
+    got = _pn(dev, idx, ks, cols, log_rows)
+    assert np.array_equal(got, oracle.pn(idx, ks, cols, log_rows))
+    if pattern != "random":
+        other = np.arange(cells, dtype=np.uint32) if pattern != "all_distinct" else np.zeros(cells, dtype=np.uint32)
+        assert np.array_equal(got, _pn(dev, other, ks, cols, log_rows))
+
+
+def test_pn_served_job_size(bellman, oracle):
+    """The served mix's permutation job at full size: 4 columns x 2^20 rows,
+    a seeded shuffle of the 2^22 cells cut into cycles of length 1..64, each
+    cycle one variable index."""
+    _, dev = bellman
+    cols, log_rows = 4, 20
+    cells = cols << log_rows
+    rng = np.random.default_rng(0xBC000005)
+    perm = rng.permutation(cells)
+    lens = rng.integers(1, 65, size=cells)
+    idx = np.zeros(cells, dtype=np.uint32)
+    pos = var = 0
+    while pos < cells:
+        L = int(lens[var])
+        idx[perm[pos:pos + L]] = var
+        pos += L
+        var += 1
+    ks = oracle.random_fr(0xBC000005, cols)
+    assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), oracle.pn(idx, ks, cols, log_rows))
+
+
 def test_pn_rejects_other_column_counts(bellman):
     _, dev = bellman
     cfg = B.generate_permutation_polynomials_configuration(dev.pool, dev.stream, None, None, None, 5, 4)
