@@ -101,9 +101,10 @@ hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out) {
   return hipSuccess;
 }
 
-ntt_job resolve_ntt(const bc_amd_ntt_job &in) {
-  return ntt_job{static_cast<const fe *>(in.inputs),   static_cast<fe *>(in.outputs), in.log_values_count, in.bit_reversed_inputs,
+hipError_t resolve_ntt(const bc_amd_ntt_job &in, ntt_job *out) {
+  *out = ntt_job{static_cast<const fe *>(in.inputs),   static_cast<fe *>(in.outputs), in.log_values_count, in.bit_reversed_inputs,
                  in.inverse, in.log_extension_degree, in.coset_index};
+  return ntt_job_valid(*out) ? hipSuccess : hipErrorInvalidValue;
 }
 
 namespace {
@@ -483,7 +484,8 @@ bc_error bc_amd_ntt_batch_async(const bc_amd_ntt_job *jobs, unsigned count, bc_m
     return bc_error_invalid_value;
   std::vector<ntt_job> v(count);
   for (unsigned i = 0; i < count; i++)
-    v[i] = resolve_ntt(jobs[i]);
+    if (const hipError_t e = resolve_ntt(jobs[i], &v[i]); e != hipSuccess)
+      return as_bc(e); // before the footprint is sized or anything is allocated
   return as_bc(run_in_block(to_pool(pool), to_stream(stream), ntt_run_footprint(v), [&](arena &ar) { return ntt_run(v, ar); }));
 }
 
@@ -567,8 +569,8 @@ bc_error bc_amd_plan_create(bc_amd_plan *plan, const bc_amd_msm_job *msm_jobs, u
   hipError_t e = hipSuccess;
   for (unsigned i = 0; i < msm_count && e == hipSuccess; i++)
     e = resolve_msm(msm_jobs[i], &p->msm[i]);
-  for (unsigned i = 0; i < ntt_count; i++)
-    p->ntt[i] = resolve_ntt(ntt_jobs[i]);
+  for (unsigned i = 0; i < ntt_count && e == hipSuccess; i++)
+    e = resolve_ntt(ntt_jobs[i], &p->ntt[i]);
   if (e == hipSuccess)
     e = plan_create(p);
   if (e != hipSuccess) {

@@ -13,7 +13,8 @@ namespace bc {
 
 // Fills in the tuned window width / accumulation chunk; rejects malformed jobs.
 hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out);
-ntt_job resolve_ntt(const bc_amd_ntt_job &in);
+// Converts an NTT job; rejects shapes and pointers that no entry point accepts.
+hipError_t resolve_ntt(const bc_amd_ntt_job &in, ntt_job *out);
 
 // Launches of at most the per-launch job limits (MSM also < 2^31 digit entries per launch).
 hipError_t msm_run(const std::vector<msm_job> &jobs, arena &ar);

@@ -35,6 +35,15 @@ struct ntt_job {
   unsigned coset;
 };
 
+// Shape of a transform that every entry point accepts: 2^1..2^NTT_MAX_LOG
+// elements, an extended domain of at most 2^NTT_MAX_LOG (compared without a
+// sum that could wrap), and a coset index below 2^log_ext.
+inline bool ntt_shape_valid(unsigned log_n, unsigned log_ext, unsigned coset) {
+  return log_n != 0 && log_n <= NTT_MAX_LOG && log_ext <= NTT_MAX_LOG - log_n && (log_ext == 0 || coset < (1u << log_ext));
+}
+
+inline bool ntt_job_valid(const ntt_job &q) { return ntt_shape_valid(q.log_n, q.log_ext, q.coset) && q.in && q.out; }
+
 struct ntt_tables {
   const fe *fwd, *inv, *inv_sizes;
   unsigned log_size;

@@ -388,9 +388,7 @@ hipError_t ntt_batch_device(const ntt_job *jobs, unsigned count, arena &ar) {
   unsigned max_log = 0;
   for (unsigned j = 0; j < count; j++) {
     const ntt_job &q = jobs[j];
-    if (q.log_n == 0 || q.log_n > NTT_MAX_LOG || q.log_n + q.log_ext > NTT_MAX_LOG || !q.in || !q.out)
-      return hipErrorInvalidValue;
-    if (q.log_ext && q.coset >= (1u << q.log_ext))
+    if (!ntt_job_valid(q))
       return hipErrorInvalidValue;
     max_log = std::max(max_log, q.log_n);
   }
@@ -517,7 +515,8 @@ hipError_t ntt_execute(const ntt_configuration &c) {
   hipStream_t s = to_stream(c.stream);
   hipMemPool_t pool = to_pool(c.mem_pool);
   const unsigned L = c.log_values_count;
-  if (L == 0 || L > NTT_MAX_LOG)
+  // checked before anything is allocated or copied
+  if (!ntt_shape_valid(L, c.log_extension_degree, c.coset_index) || !c.inputs || !c.outputs)
     return hipErrorInvalidValue;
   const size_t bytes = sizeof(fe) << L;
   const bool in_host = classify_pointer(c.inputs) != mem_kind::device;

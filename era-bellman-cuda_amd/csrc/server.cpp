@@ -112,9 +112,8 @@ hipError_t job_footprint(const bc_amd_job &job, request *r) {
     r->footprint = msm_footprint(r->msm);
     return hipSuccess;
   case BC_AMD_JOB_NTT: {
-    const ntt_job n = resolve_ntt(job.ntt);
-    if (n.log_n == 0 || n.log_n > NTT_MAX_LOG || n.log_n + n.log_ext > NTT_MAX_LOG || !n.in || !n.out || (n.log_ext && n.coset >= (1u << n.log_ext)))
-      return hipErrorInvalidValue;
+    ntt_job n;
+    BC_TRY(resolve_ntt(job.ntt, &n));
     r->ntt = n;
     r->footprint = ntt_footprint(n);
     return hipSuccess;

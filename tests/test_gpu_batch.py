@@ -113,6 +113,48 @@ def test_msm_batch_rejects_bad_jobs(bellman):
     dev.free(p)
 
 
+BAD_NTT_SHAPES = [
+    (20, 0xFFFFFFF0, 0),  # log_n + log_ext wraps below 2^32
+    (20, 9, 0),  # extended domain above 2^28
+    (4, 2, 4),  # coset index not below 2^log_ext
+    (0, 0, 0),
+    (29, 0, 0),
+    (100, 0, 0),
+]
+
+
+@pytest.mark.parametrize("log_n,log_ext,coset", BAD_NTT_SHAPES)
+def test_ntt_entry_points_reject_bad_shapes(bellman, log_n, log_ext, coset):
+    """ntt_execute_async, the NTT batch, plan creation and server submission
+    all answer bc_error_invalid_value for a transform shape outside 2^1..2^28
+    (extended domain included), before sizing or allocating anything."""
+    _, dev = bellman
+    p = dev.malloc(1024)
+    try:
+        cfg = B.ntt_configuration()
+        cfg.mem_pool, cfg.stream, cfg.inputs, cfg.outputs = dev.pool, dev.stream, p, p
+        cfg.log_values_count, cfg.log_extension_degree, cfg.coset_index = log_n, log_ext, coset
+        assert B.lib().ntt_execute_async(cfg) == B.bc_error_invalid_value
+        job = B.ntt_job(p, p, log_n, log_ext=log_ext, coset=coset)
+        jobs = B.array_of(B.bc_amd_ntt_job, [job])
+        assert B.lib().bc_amd_ntt_batch_async(jobs, ctypes.c_uint(1), dev.pool, dev.stream) == B.bc_error_invalid_value
+        plan = B.bc_amd_plan()
+        assert B.lib().bc_amd_plan_create(ctypes.byref(plan), None, ctypes.c_uint(0), jobs, ctypes.c_uint(1)) == B.bc_error_invalid_value
+        assert not plan.handle
+        srv = B.Server(hbm_budget_bytes=1 << 20)
+        try:
+            sj = B.bc_amd_job()
+            sj.kind = B.BC_AMD_JOB_NTT
+            sj.ntt = job
+            t = ctypes.c_ulonglong()
+            assert B.lib().bc_amd_server_submit(srv.handle, ctypes.byref(sj), ctypes.byref(t)) == B.bc_error_invalid_value
+        finally:
+            srv.close()
+        dev.sync()
+    finally:
+        dev.free(p)
+
+
 def test_msm_batch_chunk_edges(bellman, oracle, shared_bases):
     """Chunk lengths at the edges of the accumulation's layout, batched together:
     a chunk longer than the whole job (65535 entries), chunks of one entry, an
