@@ -78,6 +78,38 @@ def test_ntt_2_24_round_trip_and_sampled_evaluations(bellman, oracle):
     assert np.array_equal(z, x)
 
 
+@pytest.mark.timeout(300)
+@pytest.mark.parametrize("log_n", [26, 28])
+def test_ntt_maximum_sizes_round_trip(bellman, oracle, log_n):
+    """The largest transforms the API takes (2^28 elements, 8 GiB, as in the
+    reference's benchmark range, tests/ntt_test.cu:727-736): forward natural ->
+    bit-reversed then inverse bit-reversed -> natural on the device is the
+    identity, and output 0 is P(1) = sum x_i (Montgomery form is linear, so the
+    limb-wise sum of the stored inputs mod r is the stored output)."""
+    B, dev = bellman
+    n = 1 << log_n
+    rng = np.random.default_rng(0xBC000003 + log_n)
+    x = rng.integers(0, 1 << 32, size=(n, 8), dtype=np.uint32)
+    x[:, 7] %= 0x30644E72  # below r's top word: every value is below r
+    total = sum(int(x[:, i].sum(dtype=np.uint64)) << (32 * i) for i in range(8)) % oracle.R
+    d_x = dev.upload(x)
+    d_y = dev.malloc(32 * n)
+    try:
+        for inp, out, bitrev, inverse in ((d_x, d_y, False, False), (d_y, d_x, True, True)):
+            cfg = B.ntt_configuration()
+            cfg.mem_pool, cfg.stream, cfg.inputs, cfg.outputs = dev.pool, dev.stream, inp, out
+            cfg.log_values_count, cfg.bit_reversed_inputs, cfg.inverse = log_n, bitrev, inverse
+            B.call("ntt_execute_async", cfg)
+            if not inverse:
+                dev.sync()
+                assert oracle.from_limbs(dev.download(d_y, (1, 8)))[0] == total
+        dev.sync()
+        assert np.array_equal(dev.download(d_x, (n, 8)), x)
+    finally:
+        dev.free(d_x)
+        dev.free(d_y)
+
+
 def test_lde_assembled_in_whole_domain_bit_reversed_order(bellman, oracle):
     """Prover-side assembly of a 4x LDE at 2^16: running coset brev_2(c) for
     output block c gives, at position j of the 2^18 outputs, P(g w^brev(j)),
