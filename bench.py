@@ -674,9 +674,14 @@ def main():
         achieved = 64.0 * n / (per_launch_ms * 1e-3) / 1e9
         out.update({"ntt_ms": round(ntt_ms, 3), "ntt_inverse_ms": round(inv_ms, 3), "lde_4_cosets_ms": round(lde_ms, 3),
                     "ntt_call_gbs": round(64.0 * n / (ntt_ms * 1e-3) / 1e9, 1)})
+        traffic = pmc_traffic()
         out["roofline"] = {"bound": "hbm", "kernel": "ntt_pass_kernel (2^%d, %d launches per NTT)" % (args.log_ntt, launches),
                            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": pmc_traffic(), "launch_ms": round(per_launch_ms, 4)}
+                           "traffic": traffic, "launch_ms": round(per_launch_ms, 4)}
+        if traffic:
+            # measured HBM bytes per launch (counter summary) over this run's launch time
+            per_pass = traffic / (per_launch_ms * 1e-3) / 1e9
+            out["roofline"].update({"per_pass_gbs": round(per_pass, 1), "per_pass_frac": round(per_pass / HBM_PEAK_GBS, 4)})
         out.setdefault("parity", {})["ntt_round_trip"] = ntt_ok
     if "msm" in parts:
         msm_ms, msm_ok, tuned_ms, tuned_ok = headline_msm(lab, args.log_msm, args.steps, args.warmup)
