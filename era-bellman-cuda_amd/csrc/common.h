@@ -38,6 +38,16 @@ inline hipEvent_t to_event(bc_event e) { return static_cast<hipEvent_t>(e.handle
 
 inline bc_error as_bc(hipError_t e) { return static_cast<bc_error>(e); }
 
+// Per-device library state (field tables, twiddles, the generator comb) is
+// kept for devices 0..MAX_DEVICES-1.  device_slot() is the current device's
+// slot, or -1 when there is none in that range; callers then fail with
+// hipErrorInvalidDevice instead of sharing another device's state.
+constexpr int MAX_DEVICES = 64;
+inline int device_slot() {
+  int d = 0;
+  return hipGetDevice(&d) == hipSuccess && d >= 0 && d < MAX_DEVICES ? d : -1;
+}
+
 // Stream-ordered temporary from the caller's pool (or the device default pool
 // when the pool handle is null).  Freed with hipFreeAsync on the same stream.
 template <class T> hipError_t pool_alloc(T **ptr, size_t count, hipMemPool_t pool, hipStream_t stream) {

@@ -20,7 +20,6 @@ namespace {
 
 constexpr unsigned COMB_BYTES = 32;
 constexpr unsigned COMB_ENTRIES = 256;
-constexpr int MAX_DEVICES = 16;
 g1_affine *g_comb[MAX_DEVICES];
 std::mutex g_comb_mutex;
 
@@ -95,9 +94,9 @@ __global__ __launch_bounds__(FOLD_MAX) void msm_fold_kernel(const fold_args a, g
 
 hipError_t comb_table(g1_affine **table) {
   std::lock_guard<std::mutex> lock(g_comb_mutex);
-  int d = 0;
-  BC_TRY(hipGetDevice(&d));
-  d = d < MAX_DEVICES ? d : 0;
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
   if (!g_comb[d]) {
     // 2^(8j) G for j < 32, computed on the host
     g1_affine bases[COMB_BYTES];

@@ -13,15 +13,8 @@ namespace bc {
 
 namespace {
 
-constexpr int MAX_DEVICES = 16;
 ff_state g_ff[MAX_DEVICES];
 std::mutex g_ff_mutex;
-
-int current_device() {
-  int d = 0;
-  (void)hipGetDevice(&d);
-  return d < MAX_DEVICES ? d : 0;
-}
 
 // table entries: fine[j] = v^j, coarse[j] = v^(j << 14); one thread per entry
 __global__ void build_power_table_kernel(fe *fine, fe *coarse, fe v) {
@@ -52,13 +45,16 @@ hipError_t build_table(power_table &t, fe *storage, fe v) {
 } // namespace
 
 const ff_state *ff_current() {
-  const ff_state &s = g_ff[current_device()];
-  return s.ready ? &s : nullptr;
+  const int d = device_slot();
+  return d >= 0 && g_ff[d].ready ? &g_ff[d] : nullptr;
 }
 
 hipError_t ff_set_up_impl() {
   std::lock_guard<std::mutex> lock(g_ff_mutex);
-  ff_state &s = g_ff[current_device()];
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
+  ff_state &s = g_ff[d];
   if (s.ready)
     return hipSuccess;
   const size_t per_table = 2u << POW_FINE_BITS;
@@ -78,7 +74,10 @@ hipError_t ff_set_up_impl() {
 
 hipError_t ff_tear_down_impl() {
   std::lock_guard<std::mutex> lock(g_ff_mutex);
-  ff_state &s = g_ff[current_device()];
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
+  ff_state &s = g_ff[d];
   if (!s.ready)
     return hipSuccess;
   BC_TRY(hipDeviceSynchronize());

@@ -49,15 +49,8 @@ struct twiddle_state {
   std::vector<fe *> retired;
 };
 
-constexpr int MAX_DEVICES = 16;
 twiddle_state g_tw[MAX_DEVICES];
 std::mutex g_tw_mutex;
-
-int device_index() {
-  int d = 0;
-  (void)hipGetDevice(&d);
-  return d < MAX_DEVICES ? d : 0;
-}
 
 __global__ void twiddle_build_kernel(power_table w, fe *fwd, fe *inv, unsigned log_size) {
   const unsigned m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -337,7 +330,10 @@ fe inverse_power_of_two(unsigned L) {
 // because in-flight work may still read them.
 hipError_t ntt_prepare(unsigned max_log_n, hipStream_t stream, ntt_tables *out) {
   std::lock_guard<std::mutex> lock(g_tw_mutex);
-  twiddle_state &t = g_tw[device_index()];
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
+  twiddle_state &t = g_tw[d];
   if (!t.attributes) {
     const int bytes = (int)(sizeof(fe) << NTT_LOG_TILE);
     BC_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<false>), hipFuncAttributeMaxDynamicSharedMemorySize, bytes));
@@ -488,7 +484,10 @@ hipError_t ntt_batch_device(const ntt_job *jobs, unsigned count, arena &ar) {
 
 hipError_t ntt_tear_down_impl() {
   std::lock_guard<std::mutex> lock(g_tw_mutex);
-  twiddle_state &t = g_tw[device_index()];
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
+  twiddle_state &t = g_tw[d];
   BC_TRY(hipDeviceSynchronize());
   for (fe *p : t.retired)
     BC_TRY(hipFree(p));
