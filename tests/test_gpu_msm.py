@@ -171,3 +171,54 @@ def test_msm_copy_hooks(bellman, oracle, host_scalars, host_results):
         for p in (d_s, d_r, db):
             if p is not None:
                 dev.free(p)
+
+
+@pytest.mark.timeout(300)
+def test_msm_2_26_linearity(bellman, oracle):
+    """The reference's largest benchmarked MSM (2^26 points, window 22,
+    tests/msm_test.cu:253-260): bases [s_i]G made on the device from random
+    s_i, two random scalar vectors k1 and k2 and k = k1 + k2 (Montgomery form
+    is linear).  The folded results satisfy MSM(k) = MSM(k1) + MSM(k2), the
+    sum taken by the oracle as the MSM of the two folded points with scalars
+    1 and 1."""
+    import ctypes
+
+    B, dev = bellman
+    L = 26
+    n = 1 << L
+    rng = np.random.default_rng(0xBC000026)
+
+    def below_r():
+        x = rng.integers(0, 1 << 32, size=(n, 8), dtype=np.uint32)
+        x[:, 7] %= 0x30644E72  # below r's top word: every value is below r
+        return x
+
+    bufs = []
+    try:
+        d_s = dev.upload(below_r())
+        d_b = dev.malloc(64 * n)
+        bufs += [d_s, d_b]
+        B.call("bc_amd_generator_multiples", ctypes.c_void_p(d_s), ctypes.c_void_p(d_b), ctypes.c_uint(n), dev.stream)
+        d_k1, d_k2 = dev.upload(below_r()), dev.upload(below_r())
+        d_k = dev.malloc(32 * n)
+        bufs += [d_k1, d_k2, d_k]
+        B.call("ff_x_plus_y", ctypes.c_void_p(d_k1), ctypes.c_void_p(d_k2), ctypes.c_void_p(d_k), ctypes.c_uint(n), dev.stream)
+        res = [dev.malloc(254 * 96) for _ in range(3)]
+        bufs += res
+        for dk, dr in zip((d_k1, d_k2, d_k), res):
+            cfg = B.msm_configuration()
+            cfg.mem_pool, cfg.stream = dev.pool, dev.stream
+            cfg.bases, cfg.scalars, cfg.results = d_b, dk, dr
+            cfg.log_scalars_count = L
+            B.call("msm_execute_async", cfg)
+        d_p = dev.malloc(64 * 3)
+        bufs.append(d_p)
+        B.call("bc_amd_msm_fold", (ctypes.c_void_p * 3)(*res), ctypes.c_void_p(d_p), ctypes.c_uint(3), dev.stream)
+        dev.sync()
+        got = dev.download(d_p, (3, 16))
+        assert got[0].any() and got[1].any() and not np.array_equal(got[0], got[1])
+        both = oracle.msm(np.ascontiguousarray(got[:2]), oracle.to_mont([1, 1]), 1)
+        assert np.array_equal(oracle.to_affine(oracle.msm_fold(both))[0], got[2])
+    finally:
+        for p in bufs:
+            dev.free(p)
