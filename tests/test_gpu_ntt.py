@@ -16,7 +16,7 @@ def brev(x, bits):
     return int(format(x, "0%db" % bits)[::-1], 2) if bits else 0
 
 
-@pytest.mark.parametrize("log_n", [1, 2, 5, 8, 11, 12, 13, 16, 19])
+@pytest.mark.parametrize("log_n", [1, 2, 5, 8, 11, 12, 13, 16, 19, 20])
 @pytest.mark.parametrize("bitrev,inverse", [(False, False), (True, False), (False, True), (True, True)])
 def test_ntt_matches_oracle(bellman, oracle, log_n, bitrev, inverse):
     B, dev = bellman
