@@ -21,13 +21,13 @@
 
 #define BC_LAUNCH_CHECK() BC_TRY(hipGetLastError())
 
-// Same as BC_TRY for functions returning bc_error (HIP status values are
-// returned unchanged, cast to bc_error).
+// Same as BC_TRY in the C-ABI functions: HIP statuses are mapped to the
+// caller's status values (as_bc), bc_error statuses pass through.
 #define BC_TRY_BC(expr)                                                                                                                                        \
   do {                                                                                                                                                         \
-    const int bc_try_status_ = static_cast<int>(expr);                                                                                                         \
-    if (bc_try_status_ != 0)                                                                                                                                   \
-      return static_cast<bc_error>(bc_try_status_);                                                                                                            \
+    const bc_error bc_try_status_ = ::bc::to_bc_status(expr);                                                                                                  \
+    if (bc_try_status_ != bc_success)                                                                                                                          \
+      return bc_try_status_;                                                                                                                                   \
   } while (0)
 
 namespace bc {
@@ -36,7 +36,12 @@ inline hipStream_t to_stream(bc_stream s) { return static_cast<hipStream_t>(s.ha
 inline hipMemPool_t to_pool(bc_mem_pool p) { return static_cast<hipMemPool_t>(p.handle); }
 inline hipEvent_t to_event(bc_event e) { return static_cast<hipEvent_t>(e.handle); }
 
-inline bc_error as_bc(hipError_t e) { return static_cast<bc_error>(e); }
+// The status a caller of the reference would see for a HIP status: the
+// reference returns raw cudaError_t values (src/bellman-cuda.cu:9-361), and
+// HIP shares CUDA's numbering except for the codes remapped in runtime.cpp.
+bc_error as_bc(hipError_t e);
+inline bc_error to_bc_status(hipError_t e) { return as_bc(e); }
+inline bc_error to_bc_status(bc_error e) { return e; }
 
 // Per-device library state (field tables, twiddles, the generator comb) is
 // kept for devices 0..MAX_DEVICES-1.  device_slot() is the current device's
@@ -116,6 +121,43 @@ private:
   size_t size_, used_, peak_;
   bool bump_;
   std::vector<block> blocks_;
+};
+
+// Stream-ordered pool temporaries, streams and events of one call, released
+// on every return path (after everything the call enqueued).
+class call_resources {
+public:
+  call_resources(hipMemPool_t pool, hipStream_t s) : pool_(pool), s_(s) {}
+  ~call_resources() {
+    for (auto it = mem_.rbegin(); it != mem_.rend(); ++it)
+      (void)hipFreeAsync(*it, s_);
+    for (hipEvent_t e : events_)
+      (void)hipEventDestroy(e);
+    for (hipStream_t t : streams_)
+      (void)hipStreamDestroy(t);
+  }
+  template <class T> hipError_t get(T **p, size_t count) {
+    BC_TRY(pool_alloc(p, count, pool_, s_));
+    mem_.push_back(*p);
+    return hipSuccess;
+  }
+  hipError_t event(hipEvent_t *e) {
+    BC_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    events_.push_back(*e);
+    return hipSuccess;
+  }
+  hipError_t stream(hipStream_t *t) {
+    BC_TRY(hipStreamCreate(t)); // default priority
+    streams_.push_back(*t);
+    return hipSuccess;
+  }
+
+private:
+  hipMemPool_t pool_;
+  hipStream_t s_;
+  std::vector<void *> mem_;
+  std::vector<hipEvent_t> events_;
+  std::vector<hipStream_t> streams_;
 };
 
 enum class mem_kind { device, host_pinned, host_pageable };

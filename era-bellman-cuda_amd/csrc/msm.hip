@@ -38,6 +38,7 @@
 //                  across threads, then folded pairwise);
 //   6. gather:     254 Jacobian results per job.
 #include <algorithm>
+#include <functional>
 #include <vector>
 
 #include "batch.h"
@@ -467,9 +468,11 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *parti
 // Level-0 windows: A[bucket_begin + w*2^(c-1) + k] for k in [0, 2^(c-1)) holds
 // bucket k of window w (k = 0 and the top digit hold infinity); tops[top_begin + w]
 // holds the bucket of the window's top digit (2^(c-1), or 2^top_signed in the last window).
+// With `add`, the buckets of this launch are added to what A and tops already
+// hold (the chunks of one MSM accumulate into the same windows).
 __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *partials,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count, g1_xyzz *A,
-                                                                  g1_xyzz *tops) {
+                                                                  g1_xyzz *tops, bool add) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nb)
     return;
@@ -482,7 +485,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_tabl
   // to the window's top when that is its top digit, and the top digit's own
   // slot in the last window hands it there instead
   const bool to_top = k == 0 ? top_digit == B : k == top_digit;
-  if (k == 0 || k == top_digit)
+  if ((k == 0 || k == top_digit) && !add)
     store_xyzz(A + t, xyzz_infinity());
   if (k == 0 && !to_top)
     return;
@@ -492,6 +495,9 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_tabl
   const uint32_t d = k == 0 ? B : k;
   const uint32_t b = jd.bucket_begin + w * B + d - 1;
   const uint32_t np = piece_count[b], off = piece_offset[b];
+  g1_xyzz *dst = to_top ? tops + jd.top_begin + w : A + t;
+  if (add && np == 0)
+    return; // nothing of this launch to add
   g1_xyzz acc = xyzz_infinity();
   if (np != 0) {
     acc = load_xyzz(partials + off);
@@ -499,7 +505,9 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_tabl
       for (uint32_t q = 1; q < np; q++)
         acc = xyzz_add(acc, load_xyzz(partials + off + q));
   }
-  store_xyzz(to_top ? tops + jd.top_begin + w : A + t, acc);
+  if (add)
+    acc = xyzz_add(load_xyzz(dst), acc);
+  store_xyzz(dst, acc);
 }
 
 // 5. one split level for every job whose windows are still wider than one bit.
@@ -676,13 +684,20 @@ uint64_t msm_split_threads(unsigned c) {
   return total;
 }
 
-hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
-  hipStream_t s = ar.stream();
-  if (count == 0)
-    return hipSuccess;
-  if (count > MSM_MAX_JOBS)
-    return hipErrorInvalidValue;
+namespace {
+
+// Per-launch job table and totals of a batch (window buckets, top slots, ...).
+struct batch_layout {
   job_table jt{};
+  uint64_t entries = 0, nb = 0, chunks = 0, slots = 0;
+  uint32_t tops_total = 0, max_np = 0;
+};
+
+hipError_t layout_batch(const msm_job *jobs, unsigned count, batch_layout *out) {
+  if (count == 0 || count > MSM_MAX_JOBS)
+    return hipErrorInvalidValue;
+  batch_layout &bl = *out;
+  job_table &jt = bl.jt;
   jt.count = count;
   uint64_t entries = 0, nb64 = 0, chunks = 0, slots = 0;
   uint32_t tops_total = 0, max_np = 0;
@@ -716,7 +731,24 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   }
   if (entries >= (1ull << 31) || nb64 >= (1ull << 31) || slots >= (1ull << 32))
     return hipErrorInvalidValue;
-  const uint32_t nb = (uint32_t)nb64;
+  bl.entries = entries;
+  bl.nb = nb64;
+  bl.chunks = chunks;
+  bl.slots = slots;
+  bl.tops_total = tops_total;
+  bl.max_np = max_np;
+  return hipSuccess;
+}
+
+// Steps 1-4 for every job of the layout: the jobs' points are sorted into
+// buckets, summed, and stored (or, with `add`, added) into the level-0 window
+// array A (bl.nb buckets) and the top-digit slots `tops` (bl.tops_total).
+// Every temporary it takes from `ar` is returned before it ends.
+hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz *A, g1_xyzz *tops, bool add) {
+  hipStream_t s = ar.stream();
+  job_table &jt = bl.jt;
+  const uint64_t entries = bl.entries, chunks = bl.chunks, slots = bl.slots;
+  const uint32_t nb = (uint32_t)bl.nb, max_np = bl.max_np;
 
   // 1-2. digits, sort, bucket runs
   uint32_t *offsets = nullptr, *counts = nullptr;
@@ -792,15 +824,18 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   }
   BC_TRY(ar.put(heavy));
 
-  g1_xyzz *A = nullptr, *tops = nullptr;
-  BC_TRY(ar.get(&A, nb));
-  BC_TRY(ar.get(&tops, tops_total));
-  bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops);
+  bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops, add);
   BC_LAUNCH_CHECK();
   for (void *p : {(void *)partials, (void *)piece_offset, (void *)piece_count, (void *)meta, (void *)offsets, (void *)counts,
                   (void *)sorted})
     BC_TRY(ar.put(p));
+  return hipSuccess;
+}
 
+// Steps 5-6: the per-bit results of every job from its windows in A and tops.
+hipError_t reduce_phase(const batch_layout &bl, const msm_job *jobs, unsigned count, arena &ar, g1_xyzz *A, g1_xyzz *tops) {
+  hipStream_t s = ar.stream();
+  const job_table &jt = bl.jt;
   // 5. split levels, all jobs together
   std::vector<g1_xyzz *> cur(count), level_buffers;
   std::vector<uint32_t> width(count), X(count);
@@ -877,54 +912,180 @@ hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   BC_LAUNCH_CHECK();
   for (g1_xyzz *p : level_buffers)
     BC_TRY(ar.put(p));
-  BC_TRY(ar.put(A));
-  BC_TRY(ar.put(tops));
   return hipSuccess;
 }
 
-// Host-pointer handling of msm_execute_async (src/msm.cu:419-460): scalars and
-// results may be host memory; bases must be on the device.
-hipError_t msm_execute(const msm_configuration &cfg) {
+} // namespace
+
+hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
+  if (count == 0)
+    return hipSuccess;
+  batch_layout bl;
+  BC_TRY(layout_batch(jobs, count, &bl));
+  // the windows are taken first: they outlive every temporary of the accumulation
+  g1_xyzz *A = nullptr, *tops = nullptr;
+  BC_TRY(ar.get(&A, bl.nb));
+  BC_TRY(ar.get(&tops, bl.tops_total));
+  BC_TRY(accumulate_phase(bl, count, ar, A, tops, false));
+  BC_TRY(reduce_phase(bl, jobs, count, ar, A, tops));
+  BC_TRY(ar.put(tops));
+  BC_TRY(ar.put(A));
+  return hipSuccess;
+}
+
+namespace {
+
+// Host inputs are copied in four chunks from 2^18 points up: chunk k+1 moves
+// over PCIe while chunk k is accumulated (src/msm.cu:160-195 copies growing
+// chunks on side streams for the same purpose).
+unsigned host_chunk_log(unsigned L) { return L >= 18 ? L - 2 : L; }
+// Out-of-memory back-off: at most this many halvings of the chunk.
+constexpr unsigned MSM_BACKOFF_HALVINGS = 8;
+
+// One MSM of 2^L points accumulated chunk by chunk into the same windows, then
+// reduced once.  inputs_of(k) names chunk k's bases and scalars (and waits for
+// them to arrive); consumed(k) runs after chunk k's kernels are enqueued.
+hipError_t msm_chunked_device(const msm_job &full, unsigned chunk_log, arena &ar,
+                              const std::function<hipError_t(unsigned, const g1_affine **, const fe **)> &inputs_of,
+                              const std::function<hipError_t(unsigned)> &consumed) {
+  msm_job chunk = full;
+  chunk.log_n = chunk_log;
+  chunk.piece = msm_plain_piece(chunk_log);
+  batch_layout bl;
+  BC_TRY(layout_batch(&chunk, 1, &bl));
+  g1_xyzz *A = nullptr, *tops = nullptr;
+  BC_TRY(ar.get(&A, bl.nb));
+  BC_TRY(ar.get(&tops, bl.tops_total));
+  const unsigned K = 1u << (full.log_n - chunk_log);
+  for (unsigned k = 0; k < K; k++) {
+    BC_TRY(inputs_of(k, &bl.jt.job[0].bases, &bl.jt.job[0].scalars));
+    BC_TRY(accumulate_phase(bl, 1, ar, A, tops, k > 0));
+    BC_TRY(consumed(k));
+  }
+  BC_TRY(reduce_phase(bl, &chunk, 1, ar, A, tops));
+  BC_TRY(ar.put(tops));
+  BC_TRY(ar.put(A));
+  return hipSuccess;
+}
+
+// One attempt of msm_execute with chunks of 2^chunk_log points.  Every
+// allocation is made before any work is enqueued, so an out-of-memory return
+// leaves the stream as it was.
+hipError_t msm_execute_chunks(const msm_configuration &cfg, unsigned chunk_log, bool bases_host, bool scalars_host, bool results_host) {
   hipStream_t s = to_stream(cfg.stream);
-  hipMemPool_t pool = to_pool(cfg.mem_pool);
+  call_resources res(to_pool(cfg.mem_pool), s);
   const unsigned L = cfg.log_scalars_count;
-  if (L > 28)
-    return hipErrorInvalidValue;
-  if (classify_pointer(cfg.bases) != mem_kind::device)
-    return hipErrorInvalidValue;
-  const size_t n = (size_t)1 << L;
-  const bool scalars_host = classify_pointer(cfg.scalars) != mem_kind::device;
-  const bool results_host = classify_pointer(cfg.results) != mem_kind::device;
-  const fe *d_scalars = static_cast<const fe *>(cfg.scalars);
-  fe *tmp_scalars = nullptr;
-  if (scalars_host) {
-    BC_TRY(pool_alloc(&tmp_scalars, n, pool, s));
-    BC_TRY(hipMemcpyAsync(tmp_scalars, cfg.scalars, sizeof(fe) * n, hipMemcpyHostToDevice, s));
-    if (cfg.h2d_copy_finished.handle)
-      BC_TRY(hipEventRecord(to_event(cfg.h2d_copy_finished), s));
-    if (cfg.h2d_copy_finished_callback)
-      BC_TRY(hipLaunchHostFunc(s, cfg.h2d_copy_finished_callback, cfg.h2d_copy_finished_callback_data));
-    d_scalars = tmp_scalars;
+  const size_t chunk_n = (size_t)1 << chunk_log;
+  const unsigned K = 1u << (L - chunk_log);
+  const bool copying = bases_host || scalars_host;
+  // staging: two slots per copied input (chunk k uses slot k % 2)
+  fe *s_slot[2] = {nullptr, nullptr};
+  g1_affine *b_slot[2] = {nullptr, nullptr};
+  for (unsigned i = 0; i < std::min(K, 2u); i++) {
+    if (scalars_host)
+      BC_TRY(res.get(&s_slot[i], chunk_n));
+    if (bases_host)
+      BC_TRY(res.get(&b_slot[i], chunk_n));
   }
   g1_jacobian *d_results = static_cast<g1_jacobian *>(cfg.results);
-  g1_jacobian *tmp_results = nullptr;
-  if (results_host) {
-    BC_TRY(pool_alloc(&tmp_results, 254, pool, s));
-    d_results = tmp_results;
+  if (results_host)
+    BC_TRY(res.get(&d_results, 254));
+  const msm_job full{static_cast<const g1_affine *>(cfg.bases), static_cast<const fe *>(cfg.scalars), d_results, L, msm_window_bits(L),
+                     msm_plain_piece(L)};
+  msm_job chunk = full;
+  chunk.log_n = chunk_log;
+  chunk.piece = msm_plain_piece(chunk_log);
+  const size_t bytes = msm_footprint(chunk);
+  char *block = nullptr;
+  BC_TRY(res.get(&block, bytes));
+  arena ar = arena::bump_mode(block, bytes, s);
+
+  // Copies run on their own stream: chunk k's copy is enqueued right after
+  // chunk k - 1's kernels (so a pageable copy, which holds the host thread,
+  // overlaps them), into slot k % 2 once chunk k - 2 has been consumed.
+  hipStream_t cs = nullptr;
+  std::vector<hipEvent_t> loaded(K), freed(2);
+  auto enqueue_copy = [&](unsigned k) -> hipError_t {
+    if (k >= 2)
+      BC_TRY(hipStreamWaitEvent(cs, freed[k % 2], 0));
+    const size_t off = (size_t)k * chunk_n;
+    if (scalars_host)
+      BC_TRY(hipMemcpyAsync(s_slot[k % 2], static_cast<const fe *>(cfg.scalars) + off, sizeof(fe) * chunk_n, hipMemcpyHostToDevice, cs));
+    if (bases_host)
+      BC_TRY(hipMemcpyAsync(b_slot[k % 2], static_cast<const g1_affine *>(cfg.bases) + off, sizeof(g1_affine) * chunk_n, hipMemcpyHostToDevice, cs));
+    BC_TRY(hipEventRecord(loaded[k], cs));
+    if (k + 1 == K) {
+      // after the last copy, on the copy stream (src/msm.cu:183-191)
+      if (cfg.h2d_copy_finished.handle)
+        BC_TRY(hipEventRecord(to_event(cfg.h2d_copy_finished), cs));
+      if (cfg.h2d_copy_finished_callback)
+        BC_TRY(hipLaunchHostFunc(cs, cfg.h2d_copy_finished_callback, cfg.h2d_copy_finished_callback_data));
+    }
+    return hipSuccess;
+  };
+  if (copying) {
+    BC_TRY(res.stream(&cs));
+    for (auto &e : loaded)
+      BC_TRY(res.event(&e));
+    for (auto &e : freed)
+      BC_TRY(res.event(&e));
+    // the staging memory belongs to `s` until the copy stream is ordered after its allocation
+    hipEvent_t ready = nullptr;
+    BC_TRY(res.event(&ready));
+    BC_TRY(hipEventRecord(ready, s));
+    BC_TRY(hipStreamWaitEvent(cs, ready, 0));
+    BC_TRY(enqueue_copy(0));
   }
-  const msm_job job{static_cast<const g1_affine *>(cfg.bases), d_scalars, d_results, L, msm_window_bits(L), msm_plain_piece(L)};
-  BC_TRY(run_in_block(pool, s, msm_footprint(job), [&](arena &ar) { return msm_batch_device(&job, 1, ar); }));
+  auto inputs_of = [&](unsigned k, const g1_affine **b, const fe **sc) -> hipError_t {
+    const size_t off = (size_t)k * chunk_n;
+    if (copying)
+      BC_TRY(hipStreamWaitEvent(s, loaded[k], 0));
+    *b = bases_host ? b_slot[k % 2] : full.bases + off;
+    *sc = scalars_host ? s_slot[k % 2] : full.scalars + off;
+    return hipSuccess;
+  };
+  auto consumed = [&](unsigned k) -> hipError_t {
+    if (!copying || k + 1 >= K)
+      return hipSuccess;
+    BC_TRY(hipEventRecord(freed[k % 2], s));
+    return enqueue_copy(k + 1);
+  };
+  BC_TRY(msm_chunked_device(full, chunk_log, ar, inputs_of, consumed));
   if (results_host) {
-    BC_TRY(hipMemcpyAsync(cfg.results, tmp_results, sizeof(g1_jacobian) * 254, hipMemcpyDeviceToHost, s));
+    BC_TRY(hipMemcpyAsync(cfg.results, d_results, sizeof(g1_jacobian) * 254, hipMemcpyDeviceToHost, s));
     if (cfg.d2h_copy_finished.handle)
       BC_TRY(hipEventRecord(to_event(cfg.d2h_copy_finished), s));
     if (cfg.d2h_copy_finished_callback)
       BC_TRY(hipLaunchHostFunc(s, cfg.d2h_copy_finished_callback, cfg.d2h_copy_finished_callback_data));
-    BC_TRY(pool_free(tmp_results, s));
   }
-  if (tmp_scalars)
-    BC_TRY(pool_free(tmp_scalars, s));
   return hipSuccess;
+}
+
+} // namespace
+
+// msm_execute_async (src/msm.cu:419-460): bases, scalars and results may each
+// be device, pinned or pageable memory.  Device-resident inputs run as one
+// chunk; host inputs are copied in chunks overlapped with the accumulation of
+// the chunk before.  When the pool cannot supply the temporaries, the call is
+// planned again with chunks of half the size (as src/msm.cu:442-459 does), at
+// most MSM_BACKOFF_HALVINGS times.  Chunked and unchunked calls give the same
+// per-bit results as points (their Jacobian coordinates may differ).
+hipError_t msm_execute(const msm_configuration &cfg) {
+  const unsigned L = cfg.log_scalars_count;
+  if (L > 28 || !cfg.bases || !cfg.scalars || !cfg.results)
+    return hipErrorInvalidValue;
+  const bool bases_host = classify_pointer(cfg.bases) != mem_kind::device;
+  const bool scalars_host = classify_pointer(cfg.scalars) != mem_kind::device;
+  const bool results_host = classify_pointer(cfg.results) != mem_kind::device;
+  unsigned chunk_log = bases_host || scalars_host ? host_chunk_log(L) : L;
+  const unsigned min_log = L > MSM_BACKOFF_HALVINGS ? L - MSM_BACKOFF_HALVINGS : 0;
+  for (;;) {
+    const hipError_t e = msm_execute_chunks(cfg, chunk_log, bases_host, scalars_host, results_host);
+    if (e != hipErrorOutOfMemory || chunk_log <= min_log)
+      return e;
+    (void)hipGetLastError();
+    chunk_log--;
+  }
 }
 
 } // namespace bc

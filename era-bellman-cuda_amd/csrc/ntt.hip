@@ -509,39 +509,42 @@ hipError_t ntt_set_up_impl() {
 }
 
 // Host-pointer handling of ntt_execute_async (src/ntt.cu:155-231): inputs and
-// outputs may each be device, pinned or pageable memory.
+// outputs may each be device, pinned or pageable memory.  Temporaries are
+// released on every return path.
 hipError_t ntt_execute(const ntt_configuration &c) {
   hipStream_t s = to_stream(c.stream);
-  hipMemPool_t pool = to_pool(c.mem_pool);
   const unsigned L = c.log_values_count;
   // checked before anything is allocated or copied
   if (!ntt_shape_valid(L, c.log_extension_degree, c.coset_index) || !c.inputs || !c.outputs)
     return hipErrorInvalidValue;
+  call_resources res(to_pool(c.mem_pool), s);
   const size_t bytes = sizeof(fe) << L;
   const bool in_host = classify_pointer(c.inputs) != mem_kind::device;
   const bool out_host = classify_pointer(c.outputs) != mem_kind::device;
   fe *d_in = static_cast<fe *>(c.inputs);
-  fe *temp = nullptr;
+  fe *d_out = static_cast<fe *>(c.outputs);
+  fe *staged_in = nullptr;
+  if (in_host)
+    BC_TRY(res.get(&staged_in, (size_t)1 << L));
+  if (out_host && !in_host && !c.can_overwrite_inputs)
+    BC_TRY(res.get(&d_out, (size_t)1 << L));
+  const ntt_job probe{d_in, d_out, L, c.bit_reversed_inputs, c.inverse, c.log_extension_degree, c.coset_index};
+  const size_t block_bytes = ntt_footprint(probe);
+  char *block = nullptr;
+  BC_TRY(res.get(&block, block_bytes));
   if (in_host) {
-    BC_TRY(pool_alloc(&temp, (size_t)1 << L, pool, s));
-    BC_TRY(hipMemcpyAsync(temp, c.inputs, bytes, hipMemcpyHostToDevice, s));
+    BC_TRY(hipMemcpyAsync(staged_in, c.inputs, bytes, hipMemcpyHostToDevice, s));
     if (c.h2d_copy_finished.handle)
       BC_TRY(hipEventRecord(to_event(c.h2d_copy_finished), s));
     if (c.h2d_copy_finished_callback)
       BC_TRY(hipLaunchHostFunc(s, c.h2d_copy_finished_callback, c.h2d_copy_finished_callback_data));
-    d_in = temp;
+    d_in = staged_in;
   }
-  fe *d_out = static_cast<fe *>(c.outputs);
-  if (out_host) {
-    if (in_host || c.can_overwrite_inputs)
-      d_out = d_in;
-    else {
-      BC_TRY(pool_alloc(&temp, (size_t)1 << L, pool, s));
-      d_out = temp;
-    }
-  }
+  if (out_host && (in_host || c.can_overwrite_inputs))
+    d_out = d_in; // transform in place in the staging copy (or the caller's device inputs)
   const ntt_job job{d_in, d_out, L, c.bit_reversed_inputs, c.inverse, c.log_extension_degree, c.coset_index};
-  BC_TRY(run_in_block(pool, s, ntt_footprint(job), [&](arena &ar) { return ntt_batch_device(&job, 1, ar); }));
+  arena ar = arena::bump_mode(block, block_bytes, s);
+  BC_TRY(ntt_batch_device(&job, 1, ar));
   if (out_host) {
     BC_TRY(hipMemcpyAsync(c.outputs, d_out, bytes, hipMemcpyDeviceToHost, s));
     if (c.d2h_copy_finished.handle)
@@ -549,8 +552,6 @@ hipError_t ntt_execute(const ntt_configuration &c) {
     if (c.d2h_copy_finished_callback)
       BC_TRY(hipLaunchHostFunc(s, c.d2h_copy_finished_callback, c.d2h_copy_finished_callback_data));
   }
-  if (temp)
-    BC_TRY(pool_free(temp, s));
   return hipSuccess;
 }
 

@@ -26,6 +26,29 @@ mem_kind classify_pointer(const void *p) {
   }
 }
 
+// HIP statuses are CUDA's values (cudaError_t of CUDA 11.8, the toolkit the
+// reference builds with) with four exceptions, which are mapped to the CUDA
+// status of the same meaning; a value outside both enumerations reads as
+// cudaErrorUnknown (999).  bc_success, bc_error_invalid_value,
+// bc_error_memory_allocation and bc_error_not_ready are 0, 1, 2 and 600 in both.
+bc_error as_bc(hipError_t e) {
+  switch (e) {
+  case hipErrorInvalidChannelDescriptor: // 911 -> cudaErrorInvalidChannelDescriptor
+    return static_cast<bc_error>(20);
+  case hipErrorInvalidTexture: // 912 -> cudaErrorInvalidTexture
+    return static_cast<bc_error>(18);
+  case hipErrorRuntimeMemory: // 1052 -> cudaErrorMemoryAllocation
+    return bc_error_memory_allocation;
+  case hipErrorRuntimeOther:          // 1053 -> cudaErrorUnknown
+  case hipErrorContextAlreadyCurrent: // 202: no runtime-API counterpart
+    return static_cast<bc_error>(999);
+  default:
+    break;
+  }
+  const int v = static_cast<int>(e);
+  return static_cast<bc_error>(v >= 0 && v <= 999 ? v : 999);
+}
+
 } // namespace bc
 
 using namespace bc;

@@ -49,6 +49,7 @@ struct request {
   size_t footprint = 0;
   bc_amd_job_stats stats{};
   bool done = false;
+  bool claimed = false; // a wait has taken the ticket
 };
 
 struct batch {
@@ -197,16 +198,20 @@ public:
     return hipSuccess;
   }
 
+  // One waiter per ticket: the first wait claims it, and a second wait on the
+  // same ticket (concurrent or later) fails instead of sharing the request
+  // the first one frees.
   hipError_t wait(uint64_t ticket, bc_amd_job_stats *stats) {
     std::unique_lock<std::mutex> lk(m_);
     auto it = all_.find(ticket);
-    if (it == all_.end())
+    if (it == all_.end() || it->second->claimed)
       return hipErrorInvalidValue;
     request *r = it->second;
+    r->claimed = true;
     cv_done_.wait(lk, [&] { return r->done; });
     if (stats)
       *stats = r->stats;
-    all_.erase(it);
+    all_.erase(ticket);
     delete r;
     return hipSuccess;
   }

@@ -157,7 +157,9 @@ bc_error bc_amd_job_workspace_bytes(const bc_amd_job *job, size_t *bytes);
 bc_error bc_amd_server_create(bc_amd_server *server, size_t hbm_budget_bytes, unsigned max_in_flight);
 /* The job's buffers must stay valid until the job completes. */
 bc_error bc_amd_server_submit(bc_amd_server server, const bc_amd_job *job, unsigned long long *ticket);
-/* Blocks until the job completes; each ticket can be waited for once. */
+/* Blocks until the job completes.  Each ticket can be waited for once, by one
+   thread: a second wait on a ticket returns bc_error_invalid_value.  No wait
+   may be in progress when bc_amd_server_destroy is called. */
 bc_error bc_amd_server_wait(bc_amd_server server, unsigned long long ticket, bc_amd_job_stats *stats);
 /* Completes every submitted job, then stops the loop and frees the server. */
 bc_error bc_amd_server_destroy(bc_amd_server server);

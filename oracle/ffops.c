@@ -97,6 +97,35 @@ void or_poly_evaluate(const fe_t *in, const fe_t *point, size_t n, fe_t *out) {
   *out = acc;
 }
 
+/* sum_i x_i y_i in Fr (Montgomery in and out): the closed form of an MSM
+ * over bases [y_i]G is [sum k_i y_i]G.  Partial sums per thread, then added. */
+void or_fr_dot(const fe_t *x, const fe_t *y, size_t n, fe_t *out) {
+  fe_t total = {{0}};
+#pragma omp parallel
+  {
+    fe_t acc = {{0}};
+#pragma omp for schedule(static)
+    for (size_t i = 0; i < n; i++) {
+      fe_t t;
+      fe_mul(&FR, &t, &x[i], &y[i]);
+      fe_add(&FR, &acc, &acc, &t, 1);
+    }
+#pragma omp critical
+    fe_add(&FR, &total, &total, &acc, 1);
+  }
+  *out = total;
+}
+
+/* out[i] = s0 + i d in Fr, Montgomery form (s0, d Montgomery): the discrete
+ * logs of the progression bases of or_gen_bases_progression. */
+void or_fr_progression(const fe_t *s0, const fe_t *d, fe_t *out, size_t n) {
+  fe_t v = *s0;
+  for (size_t i = 0; i < n; i++) {
+    out[i] = v;
+    fe_add(&FR, &v, &v, d, 1);
+  }
+}
+
 static unsigned brev32(unsigned x, unsigned bits) {
   unsigned r = 0;
   for (unsigned i = 0; i < bits; i++)

@@ -131,6 +131,15 @@ void or_poly_eval(const fe_t *c, unsigned n, const fe_t *z, fe_t *out) {
   *out = acc;
 }
 
+/* P(z_j) for m points, one Horner pass per point, points spread over the
+ * OpenMP threads (the same evaluation as or_poly_eval, for sampled checks of
+ * large transforms). */
+void or_poly_eval_many(const fe_t *c, unsigned n, const fe_t *z, unsigned m, fe_t *out) {
+#pragma omp parallel for schedule(dynamic, 1)
+  for (unsigned j = 0; j < m; j++)
+    or_poly_eval(c, n, &z[j], &out[j]);
+}
+
 /* O(n^2) DFT in the same output order as or_ntt (natural input only):
  * out[brev(k)] = P(rho * omega_n^k), rho = coset ratio (1 without coset). */
 void or_dft_n2b(const fe_t *in, fe_t *out, unsigned log_n, unsigned log_ext, unsigned coset) {

@@ -16,6 +16,7 @@ void or_root_of_unity(fe_t *r, unsigned log_order, int inverse);
 void or_generator(fe_t *g, int inverse);
 void or_ntt(fe_t *x, unsigned log_n, int bit_reversed_inputs, int inverse, unsigned log_ext, unsigned coset);
 void or_poly_eval(const fe_t *c, unsigned n, const fe_t *z, fe_t *out);
+void or_poly_eval_many(const fe_t *c, unsigned n, const fe_t *z, unsigned m, fe_t *out);
 void or_dft_n2b(const fe_t *in, fe_t *out, unsigned log_n, unsigned log_ext, unsigned coset);
 
 unsigned or_msm_window_bits(unsigned log_n);
@@ -35,6 +36,8 @@ void or_grand_product(const fe_t *in, fe_t *out, size_t n);
 void or_multiply_by_powers(const fe_t *in, const fe_t *base, fe_t *out, size_t n);
 void or_batch_inverse(const fe_t *in, fe_t *out, size_t n);
 void or_poly_evaluate(const fe_t *in, const fe_t *point, size_t n, fe_t *out);
+void or_fr_dot(const fe_t *x, const fe_t *y, size_t n, fe_t *out);
+void or_fr_progression(const fe_t *s0, const fe_t *d, fe_t *out, size_t n);
 void or_powers_of_w(fe_t *target, unsigned log_degree, unsigned offset, unsigned count, int inverse, int bit_reversed);
 void or_powers_of_g(fe_t *target, unsigned log_degree, unsigned offset, unsigned count, int inverse, int bit_reversed);
 void or_omega_shift(const fe_t *values, fe_t *result, unsigned log_degree, unsigned shift, unsigned offset, unsigned count, int inverse);

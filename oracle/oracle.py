@@ -127,6 +127,15 @@ def poly_eval(coeffs, z):
     return out
 
 
+def poly_eval_many(coeffs, zs):
+    """P(z) for each Montgomery point of zs (m, 8), evaluated in parallel."""
+    c = np.ascontiguousarray(coeffs, dtype=np.uint32)
+    zz = np.ascontiguousarray(zs, dtype=np.uint32).reshape(-1, 8)
+    out = _fe(zz.shape[0])
+    lib().or_poly_eval_many(_p(c), c.shape[0], _p(zz), zz.shape[0], _p(out))
+    return out
+
+
 # ---- MSM ----
 def msm_window_bits(log_n):
     return int(lib().or_msm_window_bits(log_n))
@@ -235,6 +244,24 @@ def poly_evaluate(x, point):
     pt = np.ascontiguousarray(point, dtype=np.uint32).reshape(1, 8)
     out = _fe(1)
     lib().or_poly_evaluate(_p(x), _p(pt), ctypes.c_size_t(x.shape[0]), _p(out))
+    return out
+
+
+def fr_dot(x, y):
+    """sum x_i y_i mod r (Montgomery in, Montgomery out)."""
+    x = np.ascontiguousarray(x, dtype=np.uint32)
+    y = np.ascontiguousarray(y, dtype=np.uint32)
+    assert x.shape == y.shape
+    out = _fe(1)
+    lib().or_fr_dot(_p(x), _p(y), ctypes.c_size_t(x.shape[0]), _p(out))
+    return out
+
+
+def fr_progression(s0, d, n):
+    """(s0 + i d) mod r for i < n, Montgomery limbs; s0, d Python ints."""
+    a, b = to_mont([s0 % R]), to_mont([d % R])
+    out = _fe(n)
+    lib().or_fr_progression(_p(a), _p(b), _p(out), ctypes.c_size_t(n))
     return out
 
 

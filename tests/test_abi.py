@@ -97,3 +97,91 @@ def test_extension_struct_layouts_match_c_abi():
     mine += [B.bc_amd_job.msm.offset, B.bc_amd_job.ntt.offset, B.bc_amd_job.pn.offset, B.bc_amd_ntt_job.log_extension_degree.offset,
              B.bc_amd_job_stats.status.offset]
     assert got == mine
+
+
+REF_HEADER = "/root/reference/src/bellman-cuda.h"
+STRUCTS = ["bc_stream", "bc_event", "bc_mem_pool", "ff_grand_product_configuration", "ff_multiply_by_powers_configuration",
+           "ff_inverse_configuration", "ff_poly_evaluate_configuration", "ff_sort_u32_configuration",
+           "generate_permutation_polynomials_configuration", "msm_configuration", "ntt_configuration"]
+FIELDS = [("msm_configuration", f) for f in ("bases", "scalars", "results", "log_scalars_count", "h2d_copy_finished", "h2d_copy_finished_callback",
+                                             "h2d_copy_finished_callback_data", "d2h_copy_finished", "d2h_copy_finished_callback",
+                                             "d2h_copy_finished_callback_data")]
+FIELDS += [("ntt_configuration", f) for f in ("inputs", "outputs", "log_values_count", "bit_reversed_inputs", "inverse", "can_overwrite_inputs",
+                                              "log_extension_degree", "coset_index", "h2d_copy_finished", "h2d_copy_finished_callback",
+                                              "h2d_copy_finished_callback_data", "d2h_copy_finished", "d2h_copy_finished_callback",
+                                              "d2h_copy_finished_callback_data")]
+FIELDS += [("generate_permutation_polynomials_configuration", f) for f in ("indexes", "scalars", "target", "columns_count", "log_rows_count")]
+
+
+def _prototypes(path):
+    """name -> tuple of parameter types (parameter names and whitespace dropped)."""
+    text = re.sub(r"//[^\n]*|/\*.*?\*/", "", open(path).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^bc_error\s+(\w+)\s*\(([^)]*)\)\s*;", text, flags=re.M):
+        params = []
+        for p in m.group(2).split(","):
+            p = " ".join(p.split())
+            if p in ("", "void"):
+                continue
+            p = re.sub(r"\s*\*\s*", " * ", p).strip()
+            toks = p.split()
+            if toks[-1] not in ("*",) and len(toks) > 1:
+                toks = toks[:-1]  # drop the parameter name
+            params.append(" ".join(toks))
+        out[m.group(1)] = tuple(params)
+    return out
+
+
+def _layout_program(header):
+    src = "#include <stdio.h>\n#include <stddef.h>\n#include \"%s\"\n" % header
+    src += "typedef bc_error (*any_fn)(void);\n"
+    src += "int main(void) {\n"
+    for sname in STRUCTS:
+        src += '  printf("%%zu %%zu ", sizeof(%s), _Alignof(%s));\n' % (sname, sname)
+    for sname, f in FIELDS:
+        src += '  printf("%%zu ", offsetof(%s, %s));\n' % (sname, f)
+    src += '  printf("%d %d %d %d\\n", (int)bc_success, (int)bc_error_invalid_value, (int)bc_error_memory_allocation, (int)bc_error_not_ready);\n'
+    src += "  return 0;\n}\n"
+    return src
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HEADER), reason="the reference header is only present where the reference tree is")
+def test_prototypes_equal_the_reference_header():
+    ref, mine = _prototypes(REF_HEADER), _prototypes(HEADER)
+    assert len(ref) == 70
+    assert ref == mine
+
+
+@pytest.mark.skipif(not os.path.exists(REF_HEADER), reason="the reference header is only present where the reference tree is")
+def test_c_caller_built_against_the_reference_header_links_to_libbellman():
+    """A C program compiled against the reference's own header, taking the
+    address of every entry point it declares, links against libbellman.so and
+    runs (no GPU call); its struct sizes, alignments, field offsets and named
+    status values equal those seen through include/bellman-cuda.h."""
+    import tempfile
+
+    import bellman_amd as B
+
+    if not os.path.exists(B.LIB_PATH):
+        pytest.skip("libbellman.so not built")
+    names = sorted(_prototypes(REF_HEADER))
+    with tempfile.TemporaryDirectory() as d:
+        outs = []
+        for inc, hdr in ((os.path.dirname(REF_HEADER), "bellman-cuda.h"), (os.path.join(ROOT, "include"), "bellman-cuda.h")):
+            c = os.path.join(d, "layout.c")
+            open(c, "w").write(_layout_program(hdr))
+            exe = os.path.join(d, "layout")
+            subprocess.run(["gcc", "-std=c11", "-I", inc, c, "-o", exe], check=True)
+            outs.append(subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split())
+        assert outs[0] == outs[1]
+        caller = "#include \"bellman-cuda.h\"\n#include <stdio.h>\n"
+        caller += "static void *const entry_points[] = {\n" + ",\n".join("  (void *)&%s" % n for n in names) + "\n};\n"
+        caller += "int main(void) { printf(\"%zu\\n\", sizeof(entry_points) / sizeof(entry_points[0])); return 0; }\n"
+        c = os.path.join(d, "caller.c")
+        open(c, "w").write(caller)
+        exe = os.path.join(d, "caller")
+        libdir = os.path.dirname(B.LIB_PATH)
+        subprocess.run(["gcc", "-std=c11", "-I", os.path.dirname(REF_HEADER), c, "-o", exe, "-L", libdir, "-lbellman", "-Wl,-rpath," + libdir,
+                        "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,--no-as-needed"], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
+        assert out == "70"

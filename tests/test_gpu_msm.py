@@ -222,3 +222,69 @@ def test_msm_2_26_linearity(bellman, oracle):
     finally:
         for p in bufs:
             dev.free(p)
+
+
+def _host_buffer(B, arr, pinned):
+    """A copy of arr in pageable (numpy) or pinned (bc_malloc_host) host memory; (address, keep-alive, release)."""
+    import ctypes
+
+    a = np.ascontiguousarray(arr)
+    if not pinned:
+        return a.ctypes.data, a, lambda: None
+    p = ctypes.c_void_p()
+    B.call("bc_malloc_host", ctypes.byref(p), ctypes.c_size_t(a.nbytes))
+    ctypes.memmove(p.value, a.ctypes.data, a.nbytes)
+    return p.value, None, lambda: B.call("bc_free_host", p)
+
+
+@pytest.mark.parametrize("bases_where,scalars_where", [("pageable", "device"), ("pinned", "device"), ("device", "pinned"), ("pinned", "pageable"),
+                                                       ("pageable", "pageable")])
+def test_msm_host_inputs_chunked(bellman, oracle, bases_where, scalars_where):
+    """Host bases and/or host scalars at 2^20: the call copies them in four
+    chunks on a copy stream, each chunk's copy overlapping the accumulation of
+    the chunk before (src/msm.cu:160-195).  The per-bit results equal the
+    device-input call's as points, and the fold equals the closed form; the h2d
+    hooks fire once, after the last chunk."""
+    import ctypes
+
+    B, dev = bellman
+    L = 20
+    n = 1 << L
+    s0, d = 0xBC000091, 0x0123456789ABCDEF
+    bases = oracle.gen_bases_progression(s0, d, n)
+    k = oracle.msm_scalars(0xBC000092, n)
+    db, dk = dev.upload(bases), dev.upload(k)
+    want = msm(dev, db, k, L)
+    hooks = CopyHooks()
+    releases = []
+    try:
+        if bases_where == "device":
+            b_addr = db
+        else:
+            b_addr, keep_b, rel = _host_buffer(B, bases, bases_where == "pinned")
+            releases.append(rel)
+        if scalars_where == "device":
+            k_addr = dk
+        else:
+            k_addr, keep_k, rel = _host_buffer(B, k, scalars_where == "pinned")
+            releases.append(rel)
+        d_r = dev.malloc(254 * 96)
+        cfg = B.msm_configuration()
+        cfg.mem_pool, cfg.stream = dev.pool, dev.stream
+        cfg.bases, cfg.scalars, cfg.results, cfg.log_scalars_count = b_addr, k_addr, d_r, L
+        hooks.attach(cfg)
+        B.call("msm_execute_async", cfg)
+        dev.sync()
+        got = dev.download(d_r, (254, 24))
+        dev.free(d_r)
+        assert np.array_equal(oracle.to_affine(got), oracle.to_affine(want))
+        t = oracle.from_mont(oracle.fr_dot(k, oracle.fr_progression(s0, d, n)))[0]
+        assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], oracle.mul_generator(t))
+        assert hooks.wait_calls(1) == [CopyHooks.H2D]
+        assert hooks.events_complete()
+    finally:
+        hooks.close()
+        for rel in releases:
+            rel()
+        dev.free(db)
+        dev.free(dk)

@@ -362,3 +362,23 @@ def test_ff_vector_api_closed_forms(oracle):
     assert sh == [x * pow(w10, 3 * (7 + i), R) % R for i, x in enumerate(xs)]
     br = O.bit_reverse(O.to_mont(xs), 6)
     assert O.from_mont(br) == [xs[brev(i, 6)] for i in range(64)]
+
+
+def test_test_helpers_vs_bigint(oracle):
+    """The oracle helpers the GPU parity tests use as checkers: multi-point
+    evaluation (against a Python Horner loop), the Fr dot product of the MSM
+    closed form and the progression s0 + i d (against Python integers)."""
+    R = oracle.R
+    x = oracle.random_fr(0xBC0000C1, 77)
+    z = oracle.random_fr(0xBC0000C2, 5)
+    xs, zs = oracle.from_mont(x), oracle.from_mont(z)
+    got = oracle.from_mont(oracle.poly_eval_many(x, z))
+    for zi, g in zip(zs, got):
+        acc = 0
+        for c in reversed(xs):
+            acc = (acc * zi + c) % R
+        assert g == acc
+    y = oracle.random_fr(0xBC0000C3, 77)
+    assert oracle.from_mont(oracle.fr_dot(x, y))[0] == sum(a * b for a, b in zip(xs, oracle.from_mont(y))) % R
+    s0, d = R - 5, 0xFEDCBA9876543210FEDCBA
+    assert oracle.from_mont(oracle.fr_progression(s0, d, 300)) == [(s0 + i * d) % R for i in range(300)]
