@@ -34,9 +34,11 @@ import numpy as np  # noqa: E402
 import bellman_amd as B  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# v_mad_u64_u32 lane-operations per second measured on MI355X by a register-only
-# microbenchmark (256 CUs); the issue-rate ceiling of the MSM's 32x32->64 products.
-VALU_MAD_PEAK_T = 31.0
+# VALU issue peak of MI355X: 256 CUs x 4 SIMD16 x 2.4 GHz = 39.3 T lane-operations
+# per second (its 78.6 TFLOP/s FP64 vector rate / 2); a register-only probe of
+# v_mad_u64_u32 reached 31 T/s, shown beside it.
+VALU_ISSUE_PEAK_T = 39.3
+VALU_MAD_PROBE_T = 31.0
 R = 21888242871839275222246405745257275088548364400416034343698204186575808495617
 MSM_METRIC_LOG = 24
 
@@ -583,33 +585,95 @@ def served(lab, n_jobs, budget_gib, slots):
 
 
 # ---------------------------------------------------------------- CPU baseline (the oracle, host cores)
-def cpu_baseline(L_ntt_sample=22, L_msm_sample=20):
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def median_time(fn, reps):
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
+
+
+def cpu_baseline():
+    """The C oracle (the reference's host_math arithmetic restated) on this
+    host's cores, on bounded samples of the configurations: C1 (NTT 2^16) on 1
+    thread and all threads, median of 5; NTT 2^20 and the C2 MSM 2^20 on all
+    threads, median of 3, and on one thread (NTT 2^20 once, MSM 2^16 median of
+    3); the C5 PN job (4 x 2^20, serial restatement) median of 3.  `value` is
+    the headline step extrapolated from the all-thread 2^20 samples by n log n
+    (NTT) and n W (MSM)."""
     from oracle import oracle as O
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    x = O.random_fr(1, 1 << L_ntt_sample)
-    t0 = time.time()
-    O.ntt(x, L_ntt_sample)
-    t_ntt = time.time() - t0
-    n = 1 << L_msm_sample
+    all_threads = O.get_threads()
+    out = {"cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads_all": all_threads}
+    x16 = O.random_fr(0xBC000001, 1 << 16)
+    x20 = O.random_fr(0xBC000003, 1 << 20)
+    n = 1 << 20
     bases = O.gen_bases_progression(3, 5, n)
-    k = O.msm_scalars(7, n)
-    t0 = time.time()
-    O.msm(bases, k, L_msm_sample)
-    t_msm = time.time() - t0
-    c_s, c_24 = O.msm_window_bits(L_msm_sample), O.msm_window_bits(MSM_METRIC_LOG)
-    w_s, w_24 = (253 // c_s) + 1, (253 // c_24) + 1
-    # scaled by the dominant work: n log n butterflies (NTT), n * W mixed additions (MSM)
-    ntt24 = t_ntt * (24 * 2**24) / (L_ntt_sample * 2**L_ntt_sample)
-    msm24 = t_msm * (2**24 * w_24) / (2**L_msm_sample * w_s)
-    return {
-        "value": round((ntt24 + msm24) * 1000.0, 1),
-        "unit": "ms per step (MSM 2^24 + NTT 2^24), extrapolated from the sample",
-        "cores": threads,
+    k = O.msm_scalars(0xBC000002, n)
+    for threads in (1, all_threads):
+        O.set_threads(threads)
+        tag = "1t" if threads == 1 else "all"
+        out["c1_ntt_2e16_ms_" + tag] = round(median_time(lambda: O.ntt(x16, 16), 5) * 1e3, 2)
+        if threads == 1:
+            out["ntt_2e20_ms_1t"] = round(median_time(lambda: O.ntt(x20, 20), 1) * 1e3, 1)
+            out["msm_2e16_ms_1t"] = round(median_time(lambda: O.msm(bases[: 1 << 16], k[: 1 << 16], 16), 3) * 1e3, 1)
+        else:
+            out["ntt_2e20_ms_all"] = round(median_time(lambda: O.ntt(x20, 20), 3) * 1e3, 1)
+            out["c2_msm_2e20_ms_all"] = round(median_time(lambda: O.msm(bases, k, 20), 3) * 1e3, 1)
+    O.set_threads(all_threads)
+    idx = pn_indexes(0xBC00000F, 4, 20)
+    ks = O.random_fr(0xBC000010, 4)
+    out["c5_pn_4x2e20_ms_1t"] = round(median_time(lambda: O.pn(idx, ks, 4, 20), 3) * 1e3, 1)
+    c20, c24 = O.msm_window_bits(20), O.msm_window_bits(MSM_METRIC_LOG)
+    w20, w24 = 253 // c20 + 1, 253 // c24 + 1
+    ntt24 = out["ntt_2e20_ms_all"] * (24 * 2**24) / (20 * 2**20)
+    msm24 = out["c2_msm_2e20_ms_all"] * (2**24 * w24) / (2**20 * w20)
+    out.update({
+        "value": round(ntt24 + msm24, 1),
+        "unit": "ms per step (MSM 2^24 + NTT 2^24), extrapolated from the 2^20 samples",
+        "cores": all_threads,
         "kind": "port",
-        "sample": "C oracle (host_math restatement): NTT 2^%d on 1 thread %.2fs, MSM 2^%d with OpenMP over windows on %d threads %.2fs; "
-        "scaled by n log n and n W" % (L_ntt_sample, t_ntt, L_msm_sample, threads, t_msm),
-    }
+        "sample": "C oracle on %d threads of %s: NTT 2^20 %.0f ms and MSM 2^20 %.0f ms (medians of 3), scaled by n log n and n W; "
+        "1-thread and C1/C5 timings alongside" % (all_threads, out["cpu_model"], out["ntt_2e20_ms_all"], out["c2_msm_2e20_ms_all"]),
+    })
+    return out
+
+
+def profiled_kernel_ms(name):
+    """Average duration of `name` in the plain 2^24 MSM call, from the committed rocprofv3 kernel-trace summary."""
+    path = os.path.join(ROOT, "profiles", "msm_ntt_2e24_kernel_trace.md")
+    try:
+        for line in open(path):
+            if line.startswith("avg_ms %s:" % name):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def hbm_copy_ceiling(lab, nbytes=1 << 30, reps=5):
+    """Device-to-device copy of 1 GiB through bc_memcpy_async (the runtime's
+    copy kernel): read + written bytes per second, the achievable HBM rate."""
+    dev = lab.dev
+    a, b = dev.malloc(nbytes), dev.malloc(nbytes)
+    cp = lambda: B.call("bc_memcpy_async", ctypes.c_void_p(b), ctypes.c_void_p(a), ctypes.c_size_t(nbytes), dev.stream)
+    cp()
+    dev.sync()
+    ms = lab.time_ms(cp, reps)
+    dev.free(a)
+    dev.free(b)
+    return {"gbs": round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1), "ms": round(ms, 3), "bytes_each_way": nbytes}
 
 
 def pmc_traffic():
@@ -663,6 +727,9 @@ def main():
             B.call("pn_distribute_values", ctypes.c_void_p(src), ctypes.c_void_p(dst), ctypes.c_uint(n), ctypes.c_uint(1), lab.dev.stream)
         lab.dev.sync()
         log("copy kernel: 3 launches of pn_distribute_values, 2^24 elements (512 MiB read + 512 MiB written each)")
+    if "ntt" in parts:
+        out["hbm_copy_ceiling"] = hbm_copy_ceiling(lab)
+        log("hbm copy ceiling: %s" % out["hbm_copy_ceiling"])
     if "tune" in parts:
         out["tuning"] = autotune(lab)
     if "ntt" in parts:
@@ -671,28 +738,40 @@ def main():
         n = 1 << args.log_ntt
         launches = (args.log_ntt + 9) // 10  # passes of at most 10 stages
         per_launch_ms = ntt_ms / launches
-        achieved = 64.0 * n / (per_launch_ms * 1e-3) / 1e9
+        # algorithmic bytes: 64 n per NTT call (each 32 B element read and written
+        # once), i.e. 64 n / passes per launch of the pass kernel
+        bytes_per_launch = 64.0 * n / launches
+        achieved = bytes_per_launch / (per_launch_ms * 1e-3) / 1e9
         out.update({"ntt_ms": round(ntt_ms, 3), "ntt_inverse_ms": round(inv_ms, 3), "lde_4_cosets_ms": round(lde_ms, 3),
                     "ntt_call_gbs": round(64.0 * n / (ntt_ms * 1e-3) / 1e9, 1)})
         traffic = pmc_traffic()
         out["roofline"] = {"bound": "hbm", "kernel": "ntt_pass_kernel (2^%d, %d launches per NTT)" % (args.log_ntt, launches),
                            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                           "traffic": traffic, "launch_ms": round(per_launch_ms, 4)}
+                           "traffic": traffic, "algorithmic_bytes_per_launch": int(bytes_per_launch), "launch_ms": round(per_launch_ms, 4)}
         if traffic:
-            # measured HBM bytes per launch (counter summary) over this run's launch time
+            # counter-measured HBM bytes per launch (profiles/ summary) over this run's launch time
             per_pass = traffic / (per_launch_ms * 1e-3) / 1e9
-            out["roofline"].update({"per_pass_gbs": round(per_pass, 1), "per_pass_frac": round(per_pass / HBM_PEAK_GBS, 4)})
+            out["hbm_per_pass"] = {"gbs": round(per_pass, 1), "frac": round(per_pass / HBM_PEAK_GBS, 4),
+                                   "source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per ntt_pass_kernel launch, scaled by a copy kernel"}
         out.setdefault("parity", {})["ntt_round_trip"] = ntt_ok
     if "msm" in parts:
         msm_ms, msm_ok, tuned_ms, tuned_ok = headline_msm(lab, args.log_msm, args.steps, args.warmup)
         log("msm 2^%d: %.3f ms (closed form ok=%s), autotuned width %.3f ms (ok=%s)" % (args.log_msm, msm_ms, msm_ok, tuned_ms, tuned_ok))
         out.update({"msm_ms": round(msm_ms, 3), "msm_autotuned_ms": round(tuned_ms, 3)})
-        W = 253 // 20 + 1 if args.log_msm == 24 else None
-        if W:
+        c = 20 if args.log_msm == 24 else None
+        if c:
+            W = 253 // c + 1
             products = (1 << args.log_msm) * W * 10 * 136
             tps = products / (msm_ms * 1e-3) / 1e12
-            out["roofline_msm"] = {"bound": "valu (v_mad_u64_u32 issue)", "kernel": "whole msm_execute_async call", "achieved": round(tps, 2),
-                                   "peak": VALU_MAD_PEAK_T, "unit": "T 32x32-bit products/s", "frac": round(tps / VALU_MAD_PEAK_T, 4)}
+            out["roofline_msm"] = {"bound": "valu", "kernel": "whole msm_execute_async call", "achieved": round(tps, 2), "peak": VALU_ISSUE_PEAK_T,
+                                   "unit": "T 32x32-bit products/s", "frac": round(tps / VALU_ISSUE_PEAK_T, 4),
+                                   "mad_probe_t": VALU_MAD_PROBE_T, "frac_of_probe": round(tps / VALU_MAD_PROBE_T, 4),
+                                   "work": "2^24 x %d windows x 10 Fq multiplies x 136 products = %.3g" % (W, products)}
+            acc_ms = profiled_kernel_ms("accumulate_kernel")
+            if acc_ms:
+                at = products / (acc_ms * 1e-3) / 1e12
+                out["roofline_msm"]["accumulate_kernel"] = {"ms": acc_ms, "achieved": round(at, 2), "frac": round(at / VALU_ISSUE_PEAK_T, 4),
+                                                            "source": "rocprofv3 kernel trace summary in profiles/"}
         out.setdefault("parity", {}).update({"msm_closed_form": msm_ok, "msm_autotuned_closed_form": tuned_ok})
     if "pn" in parts:
         out["pn"] = headline_pn(lab, max(1, args.steps), args.warmup)

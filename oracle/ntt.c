@@ -19,6 +19,7 @@
  */
 #include "oracle.h"
 
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -78,28 +79,18 @@ void or_ntt(fe_t *x, unsigned log_n, int bit_reversed_inputs, int inverse, unsig
     tw[0] = FR.one;
     for (unsigned k = 1; k < half; k++)
       fe_mul(f, &tw[k], &tw[k - 1], &w);
-    if (!bit_reversed_inputs) {
-      const unsigned h = n >> (s + 1);
-      for (unsigned b = 0; b < half; b++) {
-        const fe_t *wb = &tw[brev(b, s)];
-        for (unsigned j = 0; j < h; j++) {
-          const unsigned i0 = b * 2 * h + j, i1 = i0 + h;
-          fe_t e = x[i0], o;
-          fe_mul(f, &o, &x[i1], wb);
-          fe_add(f, &x[i0], &e, &o, 1);
-          fe_sub(f, &x[i1], &e, &o, 1);
-        }
-      }
-    } else {
-      const unsigned h = half;
-      for (unsigned blk = 0; blk < n / (2 * h); blk++)
-        for (unsigned k = 0; k < h; k++) {
-          const unsigned i0 = blk * 2 * h + k, i1 = i0 + h;
-          fe_t e = x[i0], o;
-          fe_mul(f, &o, &x[i1], &tw[k]);
-          fe_add(f, &x[i0], &e, &o, 1);
-          fe_sub(f, &x[i1], &e, &o, 1);
-        }
+    /* the n/2 butterflies of a stage are independent: butterfly q of block
+     * q / h, offset q mod h, spread over the OpenMP threads */
+    const unsigned h = bit_reversed_inputs ? half : n >> (s + 1);
+#pragma omp parallel for schedule(static)
+    for (unsigned q = 0; q < n / 2; q++) {
+      const unsigned blk = q / h, j = q % h;
+      const unsigned i0 = blk * 2 * h + j, i1 = i0 + h;
+      const fe_t *w = bit_reversed_inputs ? &tw[j] : &tw[brev(blk, s)];
+      fe_t e = x[i0], o;
+      fe_mul(f, &o, &x[i1], w);
+      fe_add(f, &x[i0], &e, &o, 1);
+      fe_sub(f, &x[i1], &e, &o, 1);
     }
   }
   free(tw);
@@ -120,6 +111,11 @@ void or_ntt(fe_t *x, unsigned log_n, int bit_reversed_inputs, int inverse, unsig
   for (unsigned i = 0; i < n; i++)
     fe_reduce(f, &x[i], &x[i], 1);
 }
+
+/* OpenMP threads of the oracle's parallel loops (the CPU baseline times 1
+ * thread and all threads) */
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
 
 /* Horner evaluation sum_i c_i z^i (brute-force DFT building block) */
 void or_poly_eval(const fe_t *c, unsigned n, const fe_t *z, fe_t *out) {

@@ -12,6 +12,8 @@
 extern "C" {
 #endif
 
+void or_set_threads(int n);
+int or_get_threads(void);
 void or_root_of_unity(fe_t *r, unsigned log_order, int inverse);
 void or_generator(fe_t *g, int inverse);
 void or_ntt(fe_t *x, unsigned log_n, int bit_reversed_inputs, int inverse, unsigned log_ext, unsigned coset);

@@ -99,6 +99,14 @@ def msm_scalars(seed, n):
     return out
 
 
+def set_threads(n):
+    lib().or_set_threads(int(n))
+
+
+def get_threads():
+    return int(lib().or_get_threads())
+
+
 def root_of_unity(log_order, inverse=False):
     out = _fe(1)
     lib().or_root_of_unity(_p(out), log_order, 1 if inverse else 0)

@@ -221,6 +221,7 @@ def test_c4_batched_mix_plan_and_batch_calls(bellman, oracle, progression_bases)
         for j, L in enumerate(ntt_logs):
             ntt_call(dev, x_in[j], y_plain[j], L)
             dev.sync()
+        print("c4: plain calls done", flush=True)
 
         def jobs(r, y):
             m = [B.msm_job(db, d_k[j], r[j], L) for j, L in enumerate(msm_logs)]
@@ -246,6 +247,7 @@ def test_c4_batched_mix_plan_and_batch_calls(bellman, oracle, progression_bases)
         finally:
             B.call("bc_amd_plan_destroy", plan)
 
+        print("c4: plan replays done", flush=True)
         marr, narr = jobs(r_batch, y_batch)
         B.call("bc_amd_msm_batch_async", marr, ctypes.c_uint(64), dev.pool, dev.stream)
         B.call("bc_amd_ntt_batch_async", narr, ctypes.c_uint(32), dev.pool, dev.stream)
@@ -313,6 +315,7 @@ def test_c5_served_mix(bellman, oracle, progression_bases):
             else:
                 pn_call(dev, d_pn_idx[prm[0]], d_pn_ks, out, 4, 20)
             dev.sync()
+        print("c5: plain calls done", flush=True)
         srv = B.Server(hbm_budget_bytes=48 << 30, max_in_flight=3)
         try:
             tickets = []
@@ -328,6 +331,7 @@ def test_c5_served_mix(bellman, oracle, progression_bases):
             stats = [srv.wait(t) for t in tickets]
         finally:
             srv.close()
+        print("c5: served jobs done", flush=True)
         assert all(s.status == 0 for s in stats)
         assert max(s.batch_jobs for s in stats) > 1
         pn_exp = {}
