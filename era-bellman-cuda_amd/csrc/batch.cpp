@@ -199,12 +199,12 @@ size_t ntt_footprint(const ntt_job &j) {
 
 // pn_generate_device: the (variable, cell) pairs and their sorted copy (16 B
 // per cell), the sort's ping-pong buffer (8 B) and histograms (<= 2 B), then
-// the run-start table (4 B per possible variable index, <= 6 B per cell) and
-// the identity-value tables beside the sorted copy.
+// per sort bucket (2^bits <= 4/3 per cell) a run-start word and a split flag
+// (<= 7 B per cell) and the identity-value tables beside the sorted copy.
 size_t pn_footprint(unsigned cols, unsigned log_rows) {
   const uint64_t n = (uint64_t)cols << log_rows;
   const unsigned h = (log_rows + 1) / 2; // the identity-value tables: cols 2^(log_rows - h) + 2^h entries
-  return (size_t)(46 * n + 32 * (((uint64_t)cols << (log_rows - h)) + (1ull << h)) + (1u << 20));
+  return (size_t)(47 * n + 32 * (((uint64_t)cols << (log_rows - h)) + (1ull << h)) + (1u << 20));
 }
 
 hipError_t measure_scratch(hipStream_t s, const std::function<hipError_t(arena &)> &record, size_t *bytes) {

@@ -3,10 +3,14 @@
 // Cells are visited row-major (p = row*cols + col); a cell's key is its
 // variable index (column-major input).  A stable radix sort of (key, cell)
 // on the low ceil(log2 cols) + log_rows key bits groups the cells of each
-// variable, row-major inside the group.  Each group is a cycle: every cell
-// receives the identity value omega_rows^row * k_col of the NEXT cell of its
-// group (cyclically), except the first group (smallest key), which keeps the
-// identity.  The output is column-major, canonical Montgomery Fr.
+// variable, row-major inside the group.  Each run of equal keys in the sorted
+// order is a cycle: every cell receives the identity value
+// omega_rows^row * k_col of the NEXT cell of its run (cyclically), except the
+// first run, which keeps the identity.  (A variable index of 2^bits or more
+// is sorted by its low bits only, so its cells can form several runs, as in
+// the reference.)  The output is column-major, canonical Montgomery Fr.
+#include <algorithm>
+
 #include "batch.h"
 #include "common.h"
 #include "pn.h"
@@ -20,10 +24,17 @@ namespace {
 constexpr unsigned PN_BLOCK = 256;
 
 // entries[row*cols + col] = (cell << 32) | indexes[cell], cell = col*rows + row:
-// the variable index is the sort key, the cell rides along in the high word
-__global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, unsigned cols, unsigned log_rows) {
+// the variable index is the sort key, the cell rides along in the high word.
+// The same threads clear the split flags of the 2^bits sort buckets, and
+// thread 0 sets *first_end = cells (the first run's end, lowered later).
+__global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, unsigned cols, unsigned log_rows, uint8_t *split,
+                                    uint32_t buckets, uint32_t *first_end) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned cells_count = cols << log_rows;
+  if (p == 0)
+    *first_end = cells_count;
+  if (p < buckets)
+    split[p] = 0;
   if (p >= cells_count)
     return;
   const unsigned row = p / cols, col = p % cols;
@@ -31,15 +42,30 @@ __global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, 
   entries[p] = (uint64_t)cell << 32 | indexes[cell];
 }
 
-// start_of[v] = the first sorted position of variable v's run
-__global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of, unsigned n, uint32_t key_mask) {
+// Runs are maximal stretches of equal (full) keys in the sorted order, as the
+// reference's run-length encoding finds them (src/pn.cu:58-66).  The sort
+// orders keys by their low `bits` bits (their bucket) only, stably: a bucket
+// holds one key (one run) unless a variable index of 2^bits or more shares
+// its low bits with another, when the bucket's keys interleave and form
+// several runs.  Each bucket's first position goes to start_of[bucket] (one
+// writer per bucket); a boundary inside a bucket marks it split.
+// *first_end becomes the end of the first run: the smallest boundary after a
+// run of the first key.
+__global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of, uint8_t *split, unsigned n, uint32_t key_mask,
+                                     uint32_t *first_end) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n)
     return;
   const uint32_t *keys = reinterpret_cast<const uint32_t *>(sorted); // the low words (little-endian)
   const uint32_t k = keys[2 * (uint64_t)p], k_prev = keys[2 * (uint64_t)(p ? p - 1 : 0)];
-  if (p == 0 || k != k_prev)
-    start_of[k & key_mask] = p;
+  if (p == 0 || k != k_prev) {
+    if (p == 0 || (k & key_mask) != (k_prev & key_mask))
+      start_of[k & key_mask] = p;
+    else
+      split[k & key_mask] = 1;
+    if (p > 0 && k_prev == keys[0])
+      atomicMin(first_end, p);
+  }
 }
 
 // Identity values omega_rows^row * k_col as hi_tab[col][row >> h] *
@@ -67,16 +93,30 @@ __global__ void pn_tables_kernel(power_table w, const fe *scalars, fe *hi_tab, f
 // first position when p is the run's last (one table read per run), from
 // itself in the first run, and writes target[cell(p)] = omega^row * k_col of
 // that source cell.  A thread's loads are all independent but the last.
-__global__ void pn_generate_kernel(const fe *hi_tab, const fe *lo_tab, const uint64_t *sorted, const uint32_t *start_of, fe *target, unsigned n,
-                                   unsigned log_rows, uint32_t key_mask) {
+// The last position of a run in a split bucket finds its run's start by
+// walking back over the run (one thread per such run, time linear in its
+// length); runs in other buckets start at start_of[bucket].
+__global__ void pn_generate_kernel(const fe *hi_tab, const fe *lo_tab, const uint64_t *sorted, const uint32_t *start_of, const uint8_t *split,
+                                   fe *target, unsigned n, unsigned log_rows, uint32_t key_mask, const uint32_t *first_end) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n)
     return;
   const uint64_t here = sorted[p], next = sorted[p + 1 < n ? p + 1 : p];
-  const uint32_t k = (uint32_t)here, k_first = (uint32_t)sorted[0];
+  const uint32_t k = (uint32_t)here;
   uint64_t src = here; // the first run maps to itself
-  if (k != k_first)
-    src = p + 1 < n && (uint32_t)next == k ? next : sorted[min(start_of[k & key_mask], n - 1)]; // clamped for indexes >= 2^bits
+  if (p >= *first_end) {
+    if (p + 1 < n && (uint32_t)next == k) {
+      src = next;
+    } else if (!split[k & key_mask]) {
+      src = sorted[start_of[k & key_mask]];
+    } else {
+      const uint32_t *keys = reinterpret_cast<const uint32_t *>(sorted);
+      unsigned q = p;
+      while (q > 0 && keys[2 * (uint64_t)(q - 1)] == k)
+        q--;
+      src = sorted[q];
+    }
+  }
   const uint32_t src_cell = (uint32_t)(src >> 32), dst_cell = (uint32_t)(here >> 32);
   const uint32_t row = src_cell & ((1u << log_rows) - 1), col = src_cell >> log_rows;
   const unsigned h = pn_split(log_rows);
@@ -96,19 +136,23 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   hipStream_t s = ar.stream();
   const unsigned n = cols << L;
   const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
+  // keys are sorted on their low `bits` bits: 2^bits buckets
+  const uint32_t key_mask = (1u << bits) - 1, buckets = key_mask + 1;
   uint64_t *sorted = nullptr, *entries = nullptr;
+  uint32_t *first_end = nullptr;
+  uint8_t *split = nullptr;
+  BC_TRY(ar.get(&first_end, 1));
+  BC_TRY(ar.get(&split, buckets));
   BC_TRY(ar.get(&sorted, n));
   BC_TRY(ar.get(&entries, n)); // on top of the stack: reclaimed after the sort
-  pn_transpose_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, entries, cols, L);
+  pn_transpose_kernel<<<grid_for(std::max(n, buckets), PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, entries, cols, L, split, buckets, first_end);
   BC_LAUNCH_CHECK();
   const seg_group all{0, n, 1};
   BC_TRY(segmented_sort_kv(entries, sorted, &all, 1, bits, ar));
   BC_TRY(ar.put(entries));
-  // keys are sorted on their low `bits` bits: variable indexes below 2^bits
-  const uint32_t key_mask = (1u << bits) - 1;
   uint32_t *start_of = nullptr;
-  BC_TRY(ar.get(&start_of, (size_t)key_mask + 1));
-  pn_run_starts_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, start_of, n, key_mask);
+  BC_TRY(ar.get(&start_of, buckets));
+  pn_run_starts_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, start_of, split, n, key_mask, first_end);
   BC_LAUNCH_CHECK();
   fe *hi_tab = nullptr, *lo_tab = nullptr;
   const unsigned h = pn_split(L);
@@ -116,12 +160,14 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   BC_TRY(ar.get(&lo_tab, (size_t)1 << h));
   pn_tables_kernel<<<grid_for((1u << h) + (cols << (L - h)), PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, hi_tab, lo_tab, cols, L);
   BC_LAUNCH_CHECK();
-  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(hi_tab, lo_tab, sorted, start_of, target, n, L, key_mask);
+  pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(hi_tab, lo_tab, sorted, start_of, split, target, n, L, key_mask, first_end);
   BC_LAUNCH_CHECK();
   BC_TRY(ar.put(lo_tab));
   BC_TRY(ar.put(hi_tab));
   BC_TRY(ar.put(start_of));
   BC_TRY(ar.put(sorted));
+  BC_TRY(ar.put(split));
+  BC_TRY(ar.put(first_end));
   return hipSuccess;
 }
 

@@ -54,12 +54,12 @@ def pn_call(dev, d_idx, d_ks, d_out, cols, log_rows):
 
 
 def coset_points(oracle, L, log_ext, coset, ks):
-    """g w_{2^(L+e)}^coset w_{2^L}^k for each k, Montgomery limbs."""
+    """g w_{2^(L+e)}^coset w_{2^L}^k for each k (w_{2^L}^k without a coset), Montgomery limbs."""
     R = oracle.R
     w28 = pow(7, (R - 1) >> 28, R)
     w_big = pow(w28, 1 << (28 - L - log_ext), R)
     w = pow(w28, 1 << (28 - L), R)
-    rho = 7 * pow(w_big, coset, R) % R
+    rho = 7 * pow(w_big, coset, R) % R if log_ext else 1
     return oracle.to_mont([rho * pow(w, k, R) % R for k in ks])
 
 

@@ -299,6 +299,41 @@ def test_pn_degenerate_indexes(bellman, oracle, cols, log_rows, pattern):
         assert np.array_equal(got, _pn(dev, other, ks, cols, log_rows))
 
 
+@pytest.mark.parametrize("cols,log_rows,mode", [(3, 6, "interleaved"), (4, 9, "interleaved"), (4, 8, "wide_first"), (3, 10, "two_keys_one_bucket"),
+                                                 (3, 10, "long_wide_run"), (4, 12, "random_wide")])
+def test_pn_indexes_beyond_the_sorted_bits(bellman, oracle, cols, log_rows, mode):
+    """Variable indexes of 2^bits and more (bits = 2 + log_rows): the sort
+    orders them by their low bits only, so a wide index and a narrow one with
+    the same low bits interleave and form several runs each, as in the
+    reference (src/pn.cu:43-66).  Runs are found deterministically: the
+    output equals the oracle's, twice in a row."""
+    _, dev = bellman
+    cells = cols << log_rows
+    bits = 2 + log_rows
+    rng = random.Random(0xBC0000D2 + cells)
+    if mode == "two_keys_one_bucket":
+        idx = np.full(cells, (5 << bits) + 3, dtype=np.uint32)  # a wide variable and a narrow one, same low bits
+        idx[cells // 2:] = 3
+    elif mode == "long_wide_run":
+        # the first cell's narrow index shares its low bits with every other cell's wide
+        # index: one split bucket whose second run spans all other cells
+        idx = np.full(cells, (1 << bits) + 7, dtype=np.uint32)
+        idx[0] = 7
+    elif mode == "random_wide":
+        idx = np.array([rng.randrange(1 << 31) for _ in range(cells)], dtype=np.uint32)
+        for j in rng.sample(range(cells), cells // 8):
+            idx[j] = (idx[rng.randrange(cells)] & ((1 << bits) - 1)) + (rng.randrange(4) << bits)
+    else:
+        low = [rng.randrange(16) for _ in range(cells)]
+        idx = np.array([v + (rng.randrange(3) << bits) for v in low], dtype=np.uint32)
+        if mode == "wide_first":
+            idx[0] = (2 << bits) + low[0]
+    ks = oracle.random_fr(cells + 3, cols)
+    want = oracle.pn(idx, ks, cols, log_rows)
+    assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), want)
+    assert np.array_equal(_pn(dev, idx, ks, cols, log_rows), want)
+
+
 def test_pn_served_job_size(bellman, oracle):
     """The served mix's permutation job at full size: 4 columns x 2^20 rows,
     a seeded shuffle of the 2^22 cells cut into cycles of length 1..64, each

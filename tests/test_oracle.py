@@ -334,6 +334,54 @@ def test_pn_random_cycles_is_a_permutation_rotation(oracle):
             assert out[cell] == ident[nxt]
 
 
+def _pn_reference_semantics(idx, ks, cols, log_rows):
+    """Pure-Python restatement of src/pn.cu:17-97 for small inputs: a stable
+    sort of the row-major cells by the low ceil(log2 cols) + log_rows bits of
+    their variable index, run-length encoding by the full index, each run a
+    cycle (a cell takes the next cell's identity value), the first run fixed."""
+    rows = 1 << log_rows
+    bits = (cols - 1).bit_length() + log_rows
+    w = pow(pow(7, (R - 1) >> 28, R), 1 << (28 - log_rows), R)
+    cells = [(row * cols + col, col * rows + row) for row in range(rows) for col in range(cols)]
+    order = sorted(cells, key=lambda t: (int(idx[t[1]]) & ((1 << bits) - 1), t[0]))
+    out = [None] * (cols * rows)
+    start = 0
+    run = 0
+    while start < len(order):
+        end = start + 1
+        while end < len(order) and idx[order[end][1]] == idx[order[start][1]]:
+            end += 1
+        members = [c for _, c in order[start:end]]
+        for j, cell in enumerate(members):
+            src = cell if run == 0 else members[(j + 1) % len(members)]
+            out[cell] = pow(w, src % rows, R) * ks[src // rows] % R
+        start = end
+        run += 1
+    return out
+
+
+@pytest.mark.parametrize("cols,log_rows", [(3, 3), (4, 4)])
+def test_pn_indexes_beyond_the_sorted_bits(oracle, cols, log_rows):
+    """Variable indexes of 2^bits and more (bits = ceil(log2 cols) + log_rows)
+    are sorted by their low bits only (src/pn.cu:43-50) while runs compare full
+    indexes (src/pn.cu:58-66): a wide index and a narrow one with the same low
+    bits interleave and split each other's cells into several runs; a wide
+    first index splits the first run.  The oracle follows these rules as the
+    independent restatement above does."""
+    O = oracle
+    rows = 1 << log_rows
+    bits = (cols - 1).bit_length() + log_rows
+    rng = random.Random(0xBC0000D1 + cols)
+    for trial in range(20):
+        low = [rng.randrange(4) for _ in range(cols * rows)]
+        idx = np.array([v + (rng.randrange(3) << bits) for v in low], dtype=np.uint32)
+        if trial % 2:
+            idx[0] = (1 << bits) + low[0]  # a wide index on the first row-major cell: the first run is wide
+        ks = [rng.randrange(R) for _ in range(cols)]
+        got = O.from_mont(O.pn(idx, O.to_mont(ks), cols, log_rows))
+        assert got == _pn_reference_semantics(idx, ks, cols, log_rows), trial
+
+
 # ---------------- ff vector API ----------------
 def test_ff_vector_api_closed_forms(oracle):
     O = oracle
