@@ -44,6 +44,7 @@
 #include "batch.h"
 #include "common.h"
 #include "curve.cuh"
+#include "curve29.cuh"
 #include "msm.h"
 #include "scan.h"
 #include "sort.h"
@@ -420,13 +421,14 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
     const uint32_t np = piece_count[b];
     g1_xyzz *base = partials + piece_offset[b] + (size_t)(c - chunk_offset[lo]) * HEAVY_CHUNK;
     const uint32_t left = np - (c - chunk_offset[lo]) * HEAVY_CHUNK; // sums from the chunk start
-    g1_xyzz acc = xyzz_infinity();
+    xyzz29 acc;
+    bool empty = true;
     for (uint32_t k = 0; k < HEAVY_PER_THREAD; k++) {
       const uint32_t q = threadIdx.x * HEAVY_PER_THREAD + k;
       if (q < left)
-        acc = xyzz_add(acc, load_xyzz(base + q));
+        xyzz29_accumulate(acc, empty, load_xyzz(base + q));
     }
-    sh[threadIdx.x] = acc;
+    sh[threadIdx.x] = empty ? xyzz_infinity() : xyzz29_to_xyzz(acc);
     __syncthreads();
     for (unsigned half = HEAVY_BLOCK / 2; half > 0; half >>= 1) {
       if (threadIdx.x < half)
@@ -570,21 +572,23 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
   const uint32_t first = part, step = lo ? sg.D_lo : sg.D, limit = lo ? (1u << hs) : (1u << t);
   const bool active = lo ? part < sg.D_lo : j < (1u << hs);
   auto term = [&](uint32_t q) { return src + (lo ? ((q << t) | j) : ((j << t) | q)); };
-  g1_xyzz acc = xyzz_infinity();
+  // the sum in the 29-bit-limb radix; `empty` marks infinity
+  xyzz29 acc;
+  bool empty = true;
   if (active && first < limit) {
-    acc = load_xyzz(term(first));
+    xyzz29_accumulate(acc, empty, load_xyzz(term(first)));
     uint32_t q = first + step;
     if (q < limit) {
       g1_xyzz cur = load_xyzz(term(q));
       for (q += step; q < limit; q += step) {
         const g1_xyzz next = load_xyzz(term(q));
-        acc = xyzz_add(acc, cur);
+        xyzz29_accumulate(acc, empty, cur);
         cur = next;
       }
-      acc = xyzz_add(acc, cur);
+      xyzz29_accumulate(acc, empty, cur);
     }
   }
-  store_xyzz(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, acc);
+  store_xyzz(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
 }
 
 // out[i] += out[i + half] for i < half, per segment

@@ -91,34 +91,6 @@ void h29_constants(int field, uint32_t *out) {
   }
 }
 
-// acc (4 x 8 words, 2^256 radix, below 2p; ignored when *empty) += (x2, y2):
-// the 29-limb accumulation step; returns the sum in the 2^256 radix
-void h29_madd_chain(const uint32_t *points, const uint32_t *neg, int count, uint32_t *out, int *out_empty) {
-  xyzz29 acc{};
-  bool empty = true;
-  for (int k = 0; k < count; k++) {
-    fe x, y;
-    for (int i = 0; i < 8; i++) {
-      x.v[i] = points[16 * k + i];
-      y.v[i] = points[16 * k + 8 + i];
-    }
-    if (neg[k])
-      y = neg2<FqParams>(y);
-    if (empty) {
-      acc = xyzz29_from_affine(x, y);
-      empty = false;
-    } else {
-      xyzz29_add_affine(acc, empty, x, y);
-    }
-  }
-  *out_empty = empty;
-  const g1_xyzz r = empty ? xyzz_infinity() : xyzz29_to_xyzz(acc);
-  const fe *c[4] = {&r.x, &r.y, &r.zz, &r.zzz};
-  for (int k = 0; k < 4; k++)
-    for (int i = 0; i < 8; i++)
-      out[8 * k + i] = c[k]->v[i];
-}
-
 // sum of `count` XYZZ points (4 x 8 words each, 2^256 radix, below 2p;
 // ZZ = 0 is infinity) through xyzz29_accumulate, returned in the 2^256 radix
 void h29_add_chain(const uint32_t *points, int count, uint32_t *out, int *out_empty) {

@@ -177,55 +177,6 @@ def _affine_add(a, b):
     return x3, (lam * (x1 - x3) - y1) % P
 
 
-def test_bucket_sums_against_affine_arithmetic(h29, oracle):
-    """Chains of mixed additions through the 29-limb accumulation step: random
-    points, a repeated point (the doubling path), a point and its negation
-    (infinity mid-chain) and points given as lazy values (+p): the XYZZ sum,
-    back in the 2^256 radix, is the affine sum computed with Python integers."""
-    rng = random.Random(0xBC2904)
-    s = [rng.randrange(1, R) for _ in range(24)]
-    pts = oracle.gen_bases(s)  # affine Montgomery (2^256 radix), canonical
-    mont_inv = pow(1 << 256, -1, P)
-
-    def decode(pt):
-        return wvalue(pt[:8]) * mont_inv % P, wvalue(pt[8:16]) * mont_inv % P
-
-    chains = [
-        ([0, 1, 2, 3, 4, 5, 6, 7], [0, 0, 1, 0, 1, 1, 0, 0], []),
-        ([3, 3], [0, 0], []),            # acc == base: doubling
-        ([4, 4, 9], [0, 1, 0], []),      # acc == -base: infinity, then a new point
-        ([5, 6, 5, 6], [0, 0, 1, 1], []),  # back to infinity at the end
-        (list(range(24)), [i % 3 == 0 for i in range(24)], [2, 7, 11]),  # some coordinates given as value + p
-        ([8, 9, 10, 8], [0, 0, 0, 0], []),
-    ]
-    for idx, neg, lazy in chains:
-        raw = []
-        for j, i in enumerate(idx):
-            pt = pts[i].copy()
-            if j in lazy:
-                pt[:8] = words(wvalue(pt[:8]) + P)
-            raw.append(pt)
-        flat = np.ascontiguousarray(np.stack(raw), dtype=np.uint32)
-        negs = np.array([1 if v else 0 for v in neg], dtype=np.uint32)
-        out = np.zeros(32, dtype=np.uint32)
-        empty = ctypes.c_int()
-        h29.h29_madd_chain(p_(flat), p_(negs), len(idx), p_(out), ctypes.byref(empty))
-        want = None
-        for i, ng in zip(idx, neg):
-            x, y = decode(pts[i])
-            want = _affine_add(want, (x, (P - y) % P if ng else y))
-        if want is None:
-            assert empty.value == 1
-            continue
-        assert empty.value == 0
-        X, Y, ZZ, ZZZ = (wvalue(out[8 * k:8 * k + 8]) * mont_inv % P for k in range(4))
-        for k in range(4):
-            assert wvalue(out[8 * k:8 * k + 8]) < 2 * P
-        assert ZZ and pow(ZZ, 3, P) == pow(ZZZ, 2, P)
-        assert X * pow(ZZ, -1, P) % P == want[0]
-        assert Y * pow(ZZZ, -1, P) % P == want[1]
-
-
 def test_xyzz_sums_against_affine_arithmetic(h29, oracle):
     """Chains of general XYZZ additions (the bucket-reduction step): points in
     random XYZZ representations (x = X/ZZ, y = Y/ZZZ with random Z, some

@@ -374,7 +374,7 @@ def test_msm_backs_off_when_the_pool_cannot_hold_one_block(bellman, oracle, prog
     free_want = None
     pool = B.bc_mem_pool()
     B.call("bc_mem_pool_create", ctypes.byref(pool), ctypes.c_int(0))
-    dummy = ctypes.c_void_p()
+    dummies = []
     try:
         msm_call(dev, db, dk, dr, L)
         dev.sync()
@@ -390,10 +390,28 @@ def test_msm_backs_off_when_the_pool_cannot_hold_one_block(bellman, oracle, prog
             fps.append(fp.value)
         assert fps[1] < fps[0]
         free_want = (fps[0] + fps[1]) // 2
-        free, total = ctypes.c_size_t(), ctypes.c_size_t()
-        B.call("bc_mem_get_info", ctypes.byref(free), ctypes.byref(total))
-        if free.value > free_want:
-            B.call("bc_malloc", ctypes.byref(dummy), ctypes.c_size_t(free.value - free_want))
+        # fill device memory (the free figure the runtime reports is not exact
+        # on every box): blocks of 1 GiB until an allocation fails, two of them
+        # given back and refilled with 64 MiB blocks, then just enough 64 MiB
+        # blocks given back for the half-size chunk (< 64 MiB more stays free)
+        def fill(size):
+            while True:
+                blk = ctypes.c_void_p()
+                if B.lib().bc_malloc(ctypes.byref(blk), ctypes.c_size_t(size)) != B.bc_success:
+                    return
+                dummies.append((blk, size))
+
+        fill(1 << 30)
+        for _ in range(2):
+            if dummies:
+                B.call("bc_free", dummies.pop()[0])
+        fill(64 << 20)
+        ctypes.CDLL("libamdhip64.so").hipGetLastError()
+        given = 0
+        while given < free_want and dummies and dummies[-1][1] == 64 << 20:
+            B.call("bc_free", dummies.pop()[0])
+            given += 64 << 20
+        assert given >= free_want, "device memory could not be filled in 64 MiB steps"
         # the single block of the unchunked call no longer fits
         probe = ctypes.c_void_p()
         code = B.lib().bc_malloc_from_pool_async(ctypes.byref(probe), ctypes.c_size_t(fps[0]), pool, dev.stream)
@@ -409,8 +427,8 @@ def test_msm_backs_off_when_the_pool_cannot_hold_one_block(bellman, oracle, prog
         got = dev.download(dr, (254, 24))
     finally:
         dev.sync()
-        if dummy.value:
-            B.call("bc_free", dummy)
+        for blk, _ in dummies:
+            B.call("bc_free", blk)
         B.call("bc_mem_pool_destroy", pool)
         dev.free(dk)
         dev.free(dr)
