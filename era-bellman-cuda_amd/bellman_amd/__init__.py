@@ -240,7 +240,7 @@ HEADER_SYMBOLS = [
 
 # Extension entry points (include/bellman-amd.h) with no counterpart in the reference.
 EXTENSION_SYMBOLS = [
-    "bc_amd_generator_multiples", "bc_amd_msm_fold", "bc_amd_msm_batch_async", "bc_amd_ntt_batch_async", "bc_amd_msm_get_tuning", "bc_amd_msm_set_tuning",
+    "bc_amd_generator_multiples", "bc_amd_msm_fold", "bc_amd_msm_execute_no_alloc", "bc_amd_msm_batch_async", "bc_amd_ntt_batch_async", "bc_amd_msm_get_tuning", "bc_amd_msm_set_tuning",
     "bc_amd_ntt_get_split", "bc_amd_ntt_set_split", "bc_amd_msm_autotune", "bc_amd_ntt_autotune", "bc_amd_tuning_save",
     "bc_amd_tuning_load", "bc_amd_plan_create", "bc_amd_plan_workspace_bytes", "bc_amd_plan_launch", "bc_amd_plan_destroy",
     "bc_amd_job_footprint", "bc_amd_job_workspace_bytes", "bc_amd_server_create", "bc_amd_server_submit", "bc_amd_server_wait", "bc_amd_server_destroy",

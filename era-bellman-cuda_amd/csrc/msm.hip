@@ -39,6 +39,7 @@
 //   6. gather:     254 Jacobian results per job.
 #include <algorithm>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "batch.h"
@@ -1092,13 +1093,76 @@ hipError_t msm_execute(const msm_configuration &cfg) {
   }
 }
 
+namespace {
+
+// Device memory of the no-allocation entry point: one array per GPU, held
+// from the library's load until the process ends, shared by every call on
+// that GPU (calls that overlap on one GPU race on it).  On the host the name
+// `scratch` is only a stand-in for the symbol; its device address on each
+// GPU comes from hipGetSymbolAddress, resolved once per device with that
+// device current.
+constexpr size_t MSM_NO_ALLOC_BYTES = (size_t)192 << 20;
+static __device__ __align__(256) unsigned char scratch[MSM_NO_ALLOC_BYTES];
+
+std::mutex g_scratch_mu;
+void *g_scratch[MAX_DEVICES];
+
+hipError_t scratch_address(void **ptr) {
+  const int d = device_slot();
+  if (d < 0)
+    return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[d]) {
+    void *p = nullptr;
+    BC_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[d] = p;
+  }
+  *ptr = g_scratch[d];
+  return hipSuccess;
+}
+
+// The plain call's device path (same window width, chunk length and kernels,
+// hence the same per-bit results) over the static array instead of a pool
+// block.  Bases, scalars and results must be device memory, and the job's
+// footprint must fit the array; nothing is allocated.
+hipError_t msm_execute_no_alloc(const msm_configuration &cfg) {
+  const unsigned L = cfg.log_scalars_count;
+  if (L > 28 || !cfg.bases || !cfg.scalars || !cfg.results)
+    return hipErrorInvalidValue;
+  for (const void *p : {cfg.bases, cfg.scalars, cfg.results})
+    if (classify_pointer(p) != mem_kind::device)
+      return hipErrorInvalidValue;
+  const msm_job job{static_cast<const g1_affine *>(cfg.bases), static_cast<const fe *>(cfg.scalars), static_cast<g1_jacobian *>(cfg.results), L,
+                    msm_window_bits(L), msm_plain_piece(L)};
+  if (msm_footprint(job) > MSM_NO_ALLOC_BYTES)
+    return hipErrorInvalidValue;
+  void *base = nullptr;
+  BC_TRY(scratch_address(&base));
+  arena ar = arena::bump_mode(base, MSM_NO_ALLOC_BYTES, to_stream(cfg.stream));
+  auto inputs_of = [&](unsigned, const g1_affine **b, const fe **sc) -> hipError_t {
+    *b = job.bases;
+    *sc = job.scalars;
+    return hipSuccess;
+  };
+  auto consumed = [](unsigned) -> hipError_t { return hipSuccess; };
+  return msm_chunked_device(job, L, ar, inputs_of, consumed);
+}
+
+} // namespace
+
 } // namespace bc
 
 using namespace bc;
 
 extern "C" {
 
-bc_error msm_set_up() { return bc_success; }
+// Also resolves the current device's address of the no-allocation array.
+bc_error msm_set_up() {
+  void *p = nullptr;
+  return as_bc(scratch_address(&p));
+}
+
+bc_error bc_amd_msm_execute_no_alloc(msm_configuration configuration) { return as_bc(msm_execute_no_alloc(configuration)); }
 
 bc_error msm_execute_async(msm_configuration configuration) { return as_bc(msm_execute(configuration)); }
 

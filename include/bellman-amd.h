@@ -29,6 +29,21 @@ bc_error bc_amd_generator_multiples(const void *scalars, void *bases, unsigned c
  * millisecond whatever the count. */
 bc_error bc_amd_msm_fold(const void *const *results, void *points, unsigned count, bc_stream stream);
 
+/* msm_execute_async without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * stream/graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same per-bit results as msm_execute_async for the same inputs.
+ * The temporaries live in a fixed device array of
+ * BC_AMD_MSM_NO_ALLOC_BYTES per GPU (held from the library's load until the
+ * process ends), so log_scalars_count is at most BC_AMD_MSM_NO_ALLOC_MAX_LOG;
+ * larger sizes, and bases, scalars or results outside device memory, return
+ * bc_error_invalid_value.  mem_pool and the h2d/d2h hooks are ignored.  Every
+ * call on a GPU uses that GPU's one array: calls that overlap on one GPU
+ * (different streams, or threads) race and must be serialized by the caller. */
+#define BC_AMD_MSM_NO_ALLOC_BYTES (192u << 20)
+#define BC_AMD_MSM_NO_ALLOC_MAX_LOG 17
+bc_error bc_amd_msm_execute_no_alloc(msm_configuration configuration);
+
 /* ------------------------------------------------------------------------ */
 /* Jobs.  All buffers are device memory.                                     */
 

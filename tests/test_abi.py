@@ -185,3 +185,32 @@ def test_c_caller_built_against_the_reference_header_links_to_libbellman():
                         "-Wl,-rpath-link,/opt/rocm/lib", "-Wl,--no-as-needed"], check=True)
         out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout.strip()
         assert out == "70"
+
+
+def test_no_alloc_limits_match_the_plain_call_footprints():
+    """BC_AMD_MSM_NO_ALLOC_MAX_LOG is the largest plain MSM whose temporaries
+    (bc_amd_job_footprint with the plain call's window width and chunk) fit
+    BC_AMD_MSM_NO_ALLOC_BYTES; host arithmetic only."""
+    import ctypes
+
+    import bellman_amd as B
+    from oracle import oracle as O
+
+    if not os.path.exists(B.LIB_PATH):
+        pytest.skip("libbellman.so not built")
+    text = open(EXT_HEADER).read()
+    nbytes = eval(re.search(r"#define BC_AMD_MSM_NO_ALLOC_BYTES \((\d+)u << (\d+)\)", text).expand(r"\1 << \2"))
+    max_log = int(re.search(r"#define BC_AMD_MSM_NO_ALLOC_MAX_LOG (\d+)", text).group(1))
+
+    def plain_piece(L):  # the plain call's chunk of sorted entries per thread
+        return 128 if L >= 24 else 64 if L == 23 else 32 if L >= 21 else 16 if L >= 19 else 8 if L >= 16 else 4
+
+    def footprint(L):
+        job = B.bc_amd_job()
+        job.kind = B.BC_AMD_JOB_MSM
+        job.msm = B.msm_job(1, 1, 1, L, O.msm_window_bits(L), plain_piece(L))
+        out = ctypes.c_size_t()
+        B.call("bc_amd_job_footprint", ctypes.byref(job), ctypes.byref(out))
+        return out.value
+
+    assert footprint(max_log) <= nbytes < footprint(max_log + 1)
