@@ -23,23 +23,66 @@ namespace {
 
 constexpr unsigned PN_BLOCK = 256;
 
-// entries[row*cols + col] = (cell << 32) | indexes[cell], cell = col*rows + row:
-// the variable index is the sort key, the cell rides along in the high word.
-// The same threads clear the split flags of the 2^bits sort buckets, and
-// thread 0 sets *first_end = cells (the first run's end, lowered later).
-__global__ void pn_transpose_kernel(const uint32_t *indexes, uint64_t *entries, unsigned cols, unsigned log_rows, uint8_t *split,
-                                    uint32_t buckets, uint32_t *first_end) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+// Identity values omega_rows^row * k_col as hi_tab[col][row >> h] *
+// lo_tab[row mod 2^h], h = pn_split(log_rows): hi_tab holds
+// k_col * omega_rows^(hi 2^h), lo_tab omega_rows^lo (omega_rows^row =
+// omega_{2^28}^(row 2^(28 - log_rows))).
+__host__ __device__ constexpr unsigned pn_split(unsigned log_rows) { return (log_rows + 1) / 2; }
+
+// One launch before the sort.  Blocks [0, tiles) each take one sort tile of
+// SEG_TILE row-major positions: entries[row*cols + col] = (cell << 32) |
+// indexes[cell], cell = col*rows + row (the variable index is the sort key,
+// the cell rides along in the high word), and the tile's counts of the first
+// sort pass's digit (key bits [0, first_bits)) go to column `tile` of
+// first_table, laid out [digit][tile].  The remaining blocks clear the split
+// flags of the 2^bits sort buckets and build the identity tables; thread 0
+// sets *first_end = cells (the first run's end, lowered later).
+__global__ __launch_bounds__(PN_BLOCK) void pn_prepare_kernel(const uint32_t *indexes, uint64_t *entries, uint32_t *first_table, unsigned first_bits,
+                                                              unsigned cols, unsigned log_rows, uint32_t tiles, uint8_t *split, uint32_t buckets,
+                                                              uint32_t *first_end, power_table w, const fe *scalars, fe *hi_tab, fe *lo_tab) {
+  extern __shared__ uint32_t hist[];
   const unsigned cells_count = cols << log_rows;
-  if (p == 0)
-    *first_end = cells_count;
-  if (p < buckets)
-    split[p] = 0;
-  if (p >= cells_count)
+  if (blockIdx.x < tiles) {
+    const unsigned digits = 1u << first_bits;
+    for (unsigned d = threadIdx.x; d < digits; d += PN_BLOCK)
+      hist[d] = 0;
+    __syncthreads();
+    // all of a thread's index loads are issued before the first is used
+    constexpr unsigned PER_THREAD = SEG_TILE / PN_BLOCK;
+    const unsigned p0 = blockIdx.x * SEG_TILE + threadIdx.x;
+    uint32_t key[PER_THREAD], cell[PER_THREAD];
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++) {
+      const unsigned p = p0 + r * PN_BLOCK;
+      const unsigned row = p / cols, col = p % cols;
+      cell[r] = (col << log_rows) + row;
+      key[r] = p < cells_count ? indexes[cell[r]] : 0u;
+    }
+#pragma unroll
+    for (unsigned r = 0; r < PER_THREAD; r++) {
+      const unsigned p = p0 + r * PN_BLOCK;
+      if (p < cells_count) {
+        entries[p] = (uint64_t)cell[r] << 32 | key[r];
+        atomicAdd(&hist[key[r] & (digits - 1)], 1u);
+      }
+    }
+    __syncthreads();
+    for (unsigned d = threadIdx.x; d < digits; d += PN_BLOCK)
+      first_table[(uint64_t)d * tiles + blockIdx.x] = hist[d];
     return;
-  const unsigned row = p / cols, col = p % cols;
-  const unsigned cell = (col << log_rows) + row;
-  entries[p] = (uint64_t)cell << 32 | indexes[cell];
+  }
+  const unsigned i = (blockIdx.x - tiles) * PN_BLOCK + threadIdx.x;
+  if (i == 0)
+    *first_end = cells_count;
+  if (i < buckets)
+    split[i] = 0;
+  const unsigned h = pn_split(log_rows), n_lo = 1u << h, n_hi = 1u << (log_rows - h), shift = POW_LOG_RANGE - log_rows;
+  if (i < n_lo) {
+    store_fe(lo_tab + i, table_pow(w, i << shift));
+  } else if (i - n_lo < cols * n_hi) {
+    const unsigned j = i - n_lo; // j = col n_hi + hi
+    store_fe(hi_tab + j, mul<FrParams>(table_pow(w, ((j % n_hi) << h) << shift), load_fe(scalars + j / n_hi)));
+  }
 }
 
 // Runs are maximal stretches of equal (full) keys in the sorted order, as the
@@ -66,24 +109,6 @@ __global__ void pn_run_starts_kernel(const uint64_t *sorted, uint32_t *start_of,
     if (p > 0 && k_prev == keys[0])
       atomicMin(first_end, p);
   }
-}
-
-// Identity values omega_rows^row * k_col as hi_tab[col][row >> h] *
-// lo_tab[row mod 2^h], h = pn_split(log_rows): hi_tab holds
-// k_col * omega_rows^(hi 2^h), lo_tab omega_rows^lo (omega_rows^row =
-// omega_{2^28}^(row 2^(28 - log_rows))).
-__host__ __device__ constexpr unsigned pn_split(unsigned log_rows) { return (log_rows + 1) / 2; }
-
-__global__ void pn_tables_kernel(power_table w, const fe *scalars, fe *hi_tab, fe *lo_tab, unsigned cols, unsigned log_rows) {
-  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned h = pn_split(log_rows), n_lo = 1u << h, n_hi = 1u << (log_rows - h), shift = POW_LOG_RANGE - log_rows;
-  if (i < n_lo) {
-    store_fe(lo_tab + i, table_pow(w, i << shift));
-    return;
-  }
-  const unsigned j = i - n_lo; // j = col n_hi + hi
-  if (j < cols * n_hi)
-    store_fe(hi_tab + j, mul<FrParams>(table_pow(w, ((j % n_hi) << h) << shift), load_fe(scalars + j / n_hi)));
 }
 
 // In the reference's direction (src/pn_kernels.cu:89-121), sorted position j
@@ -138,34 +163,38 @@ hipError_t pn_generate_device(const uint32_t *indexes, const fe *scalars, fe *ta
   const unsigned bits = 2 + L; // ceil(log2 cols) for 3 or 4 columns
   // keys are sorted on their low `bits` bits: 2^bits buckets
   const uint32_t key_mask = (1u << bits) - 1, buckets = key_mask + 1;
+  const seg_group all{0, n, 1};
+  seg_first_pass fp;
+  BC_TRY(segmented_sort_first_pass(&all, 1, bits, &fp));
+  const uint32_t tiles = (n + SEG_TILE - 1) / SEG_TILE;
+  const unsigned h = pn_split(L);
+  const uint32_t table_threads = std::max<uint32_t>(buckets, (1u << h) + (cols << (L - h)));
   uint64_t *sorted = nullptr, *entries = nullptr;
-  uint32_t *first_end = nullptr;
+  uint32_t *first_end = nullptr, *first_table = nullptr, *start_of = nullptr;
   uint8_t *split = nullptr;
+  fe *hi_tab = nullptr, *lo_tab = nullptr;
   BC_TRY(ar.get(&first_end, 1));
   BC_TRY(ar.get(&split, buckets));
+  BC_TRY(ar.get(&hi_tab, (size_t)cols << (L - h)));
+  BC_TRY(ar.get(&lo_tab, (size_t)1 << h));
   BC_TRY(ar.get(&sorted, n));
+  BC_TRY(ar.get(&first_table, fp.table_entries));
   BC_TRY(ar.get(&entries, n)); // on top of the stack: reclaimed after the sort
-  pn_transpose_kernel<<<grid_for(std::max(n, buckets), PN_BLOCK), PN_BLOCK, 0, s>>>(indexes, entries, cols, L, split, buckets, first_end);
+  pn_prepare_kernel<<<tiles + grid_for(table_threads, PN_BLOCK), PN_BLOCK, sizeof(uint32_t) << fp.bits, s>>>(
+      indexes, entries, first_table, fp.bits, cols, L, tiles, split, buckets, first_end, ff->w, scalars, hi_tab, lo_tab);
   BC_LAUNCH_CHECK();
-  const seg_group all{0, n, 1};
-  BC_TRY(segmented_sort_kv(entries, sorted, &all, 1, bits, ar));
+  BC_TRY(segmented_sort_kv(entries, sorted, &all, 1, bits, ar, first_table));
   BC_TRY(ar.put(entries));
-  uint32_t *start_of = nullptr;
+  BC_TRY(ar.put(first_table));
   BC_TRY(ar.get(&start_of, buckets));
   pn_run_starts_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(sorted, start_of, split, n, key_mask, first_end);
   BC_LAUNCH_CHECK();
-  fe *hi_tab = nullptr, *lo_tab = nullptr;
-  const unsigned h = pn_split(L);
-  BC_TRY(ar.get(&hi_tab, (size_t)cols << (L - h)));
-  BC_TRY(ar.get(&lo_tab, (size_t)1 << h));
-  pn_tables_kernel<<<grid_for((1u << h) + (cols << (L - h)), PN_BLOCK), PN_BLOCK, 0, s>>>(ff->w, scalars, hi_tab, lo_tab, cols, L);
-  BC_LAUNCH_CHECK();
   pn_generate_kernel<<<grid_for(n, PN_BLOCK), PN_BLOCK, 0, s>>>(hi_tab, lo_tab, sorted, start_of, split, target, n, L, key_mask, first_end);
   BC_LAUNCH_CHECK();
-  BC_TRY(ar.put(lo_tab));
-  BC_TRY(ar.put(hi_tab));
   BC_TRY(ar.put(start_of));
   BC_TRY(ar.put(sorted));
+  BC_TRY(ar.put(lo_tab));
+  BC_TRY(ar.put(hi_tab));
   BC_TRY(ar.put(split));
   BC_TRY(ar.put(first_end));
   return hipSuccess;
