@@ -339,7 +339,8 @@ def batched(lab, reps):
             ntt_call(lab, x_in[j], y_plain[j], L)
             dev.sync()
 
-    one_at_a_time()  # warm-up (pool growth, twiddles)
+    for _ in range(2):  # warm-up (pool growth, twiddles, first launches)
+        one_at_a_time()
     plain_s = []
     for _ in range(reps):
         t = time.perf_counter()

@@ -397,53 +397,82 @@ struct plan_state {
   std::vector<msm_job> msm;
   std::vector<ntt_job> ntt;
   void *workspace = nullptr;
-  size_t bytes = 0;
+  size_t bytes = 0;     // the whole workspace
+  size_t msm_bytes = 0; // its MSM region; the NTT region follows
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t last = nullptr;
 };
 
-hipError_t record(const plan_state &p, arena &ar) {
-  BC_TRY(msm_run(p.msm, ar));
-  return ntt_run(p.ntt, ar);
-}
-
-// Captures the plan's launch sequence on `s` with arena `ar`.
-hipError_t capture(const plan_state &p, arena &ar, hipStream_t s, hipGraph_t *graph) {
-  BC_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  const hipError_t e = record(p, ar);
-  hipGraph_t g = nullptr;
-  const hipError_t end = hipStreamEndCapture(s, &g);
-  if (e != hipSuccess || end != hipSuccess) {
-    if (g)
+// Captures the plan on `s`: the MSM batch on `s`, the NTT batch on a branch
+// (`s2`, forked and joined by events) so that a replay may run the two
+// concurrently; each has its own region of the workspace.
+hipError_t capture(const plan_state &p, hipStream_t s, hipStream_t s2, hipGraph_t *graph) {
+  hipEvent_t fork = nullptr, join = nullptr;
+  BC_TRY(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+  hipError_t e = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    char *base = static_cast<char *>(p.workspace);
+    arena msm_ar = arena::bump_mode(base, p.msm_bytes, s);
+    arena ntt_ar = arena::bump_mode(base + p.msm_bytes, p.bytes - p.msm_bytes, s2);
+    e = hipEventRecord(fork, s);
+    if (e == hipSuccess)
+      e = hipStreamWaitEvent(s2, fork, 0);
+    if (e == hipSuccess)
+      e = ntt_run(p.ntt, ntt_ar);
+    if (e == hipSuccess)
+      e = msm_run(p.msm, msm_ar);
+    // the branch is joined even after an error, so that the capture can end
+    const hipError_t j = hipEventRecord(join, s2);
+    const hipError_t w = j == hipSuccess ? hipStreamWaitEvent(s, join, 0) : j;
+    if (e == hipSuccess)
+      e = w;
+    hipGraph_t g = nullptr;
+    const hipError_t end = hipStreamEndCapture(s, &g);
+    if (e == hipSuccess)
+      e = end;
+    if (e == hipSuccess)
+      *graph = g;
+    else if (g)
       (void)hipGraphDestroy(g);
-    return e != hipSuccess ? e : end;
   }
-  *graph = g;
-  return hipSuccess;
+  (void)hipEventDestroy(fork);
+  if (join)
+    (void)hipEventDestroy(join);
+  return e;
 }
 
 hipError_t plan_create(plan_state *p) {
   unsigned max_log = 0;
   for (const ntt_job &j : p->ntt)
     max_log = std::max(max_log, j.log_n);
-  hipStream_t s = nullptr;
+  hipStream_t s = nullptr, s2 = nullptr;
   BC_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  hipError_t e = max_log ? ntt_prepare(max_log, s, nullptr) : hipSuccess;
-  // First capture only sizes the workspace: its kernels see placeholder
-  // addresses and the graph is destroyed without being launched.
+  hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+  if (e == hipSuccess && max_log)
+    e = ntt_prepare(max_log, s, nullptr);
+  // The sizing captures see placeholder addresses; their graphs are destroyed
+  // without being launched.
+  size_t ntt_bytes = 0;
   if (e == hipSuccess)
-    e = measure_scratch(s, [&](arena &ar) { return record(*p, ar); }, &p->bytes);
+    e = measure_scratch(s, [&](arena &ar) { return msm_run(p->msm, ar); }, &p->msm_bytes);
   if (e == hipSuccess)
-    e = hipMalloc(&p->workspace, std::max<size_t>(p->bytes, 256));
+    e = measure_scratch(s, [&](arena &ar) { return ntt_run(p->ntt, ar); }, &ntt_bytes);
   if (e == hipSuccess) {
-    arena ws = arena::bump_mode(p->workspace, p->bytes, s);
-    e = capture(*p, ws, s, &p->graph);
+    p->msm_bytes = (p->msm_bytes + 255) & ~size_t(255);
+    p->bytes = p->msm_bytes + ntt_bytes;
+    e = hipMalloc(&p->workspace, std::max<size_t>(p->bytes, 256));
   }
+  if (e == hipSuccess)
+    e = capture(*p, s, s2, &p->graph);
   if (e == hipSuccess)
     e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
   if (e == hipSuccess)
     e = hipEventCreateWithFlags(&p->last, hipEventDisableTiming);
+  if (s2)
+    (void)hipStreamDestroy(s2);
   (void)hipStreamDestroy(s);
   return e;
 }

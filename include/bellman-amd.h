@@ -117,7 +117,9 @@ bc_error bc_amd_tuning_load(const char *path);
 /* ------------------------------------------------------------------------ */
 /* Plans: a fixed set of MSM and NTT jobs (buffers included) captured once   */
 /* into a hipGraph over a workspace owned by the plan.  Each launch replays  */
-/* the whole set with no allocation and no host-side work per job.           */
+/* the whole set with no allocation and no host-side work per job.  The MSM  */
+/* and NTT batches are parallel branches of the graph and may run            */
+/* concurrently: no job of a plan may write a buffer another job reads.      */
 
 typedef struct {
   void *handle;
