@@ -10,9 +10,12 @@ ABI with inputs resident in HBM.  Beside it:
            (4 x 2^20) jobs through the continuous-batching server: capacity,
            p50/p99 latency under Poisson arrivals, and the one-at-a-time FIFO
            under the same arrivals.
-Every batched and served result is checked against its plain-call result (NTT
-and PN bytes, MSM folded point) and every MSM fold against the closed form
-[sum k_i s_i]G, on the device.  Inputs are synthetic and seeded (splitmix64).
+The batched and served MSMs run over fixed-base tables of the resident bases
+(built once, outside the timed regions; the same forms with the MSMs over the
+bases themselves are reported beside them).  Every batched and served result
+is checked against its plain-call result (NTT and PN bytes, MSM folded point)
+and every MSM fold against the closed form [sum k_i s_i]G, on the device.
+Inputs are synthetic and seeded (splitmix64).
 Prints one JSON line; progress goes to stderr.
 """
 import argparse
@@ -199,6 +202,31 @@ def pn_call(lab, d_idx, d_ks, d_out, cols, log_rows):
     B.call("pn_generate_permutation_polynomials", cfg)
 
 
+# Fixed-base tables (bc_amd_msm_fixed_base_table) for MSM jobs over resident
+# bases: three widths cover 2^12..2^22; the width per job size was the fastest
+# of a per-size sweep of batches of equal jobs on MI355X, restricted to them.
+FIXED_BASE_WIDTHS = (14, 17, 20)
+
+
+def fixed_base_width(log_n):
+    return 14 if log_n <= 15 else (17 if log_n <= 19 else 20)
+
+
+def fixed_base_tables(lab, d_bases, log_count):
+    """One table per width of FIXED_BASE_WIDTHS over 2^log_count bases: {c: device pointer}, seconds, bytes."""
+    dev = lab.dev
+    tables, nbytes = {}, 0
+    t0 = time.perf_counter()
+    for c in FIXED_BASE_WIDTHS:
+        size = ((253 // c) + 1) * (64 << log_count)
+        tables[c] = dev.malloc(size)
+        nbytes += size
+        B.call("bc_amd_msm_fixed_base_table", ctypes.c_void_p(d_bases), ctypes.c_uint(log_count), ctypes.c_uint(c), ctypes.c_void_p(tables[c]),
+               dev.stream)
+    dev.sync()
+    return tables, time.perf_counter() - t0, nbytes
+
+
 def pn_indexes(seed, cols, log_rows):
     """Variable indexes forming random cycles of length 1..64 over the cells."""
     cells = cols << log_rows
@@ -375,12 +403,29 @@ def batched(lab, reps):
     plan_wall_s = (time.perf_counter() - t) / reps
     B.call("bc_amd_plan_destroy", plan)
 
+    # the same jobs with the MSMs over fixed-base tables of the shared bases
+    tables, table_s, table_bytes = fixed_base_tables(lab, d_b, 20)
+    r_fixed = [dev.malloc(254 * 96) for _ in msm_logs]
+    fb_jobs = [B.msm_job(tables[fixed_base_width(L)], d_k[j], r_fixed[j], L, fixed_base_width(L), 0, 20) for j, L in enumerate(msm_logs)]
+    fbarr = B.array_of(B.bc_amd_msm_job, fb_jobs)
+    plan_fb = B.bc_amd_plan()
+    B.call("bc_amd_plan_create", ctypes.byref(plan_fb), fbarr, ctypes.c_uint(len(fb_jobs)), narr, ctypes.c_uint(len(ntt_jobs)))
+    B.call("bc_amd_plan_launch", plan_fb, dev.stream)
+    dev.sync()
+    t = time.perf_counter()
+    for _ in range(reps):
+        B.call("bc_amd_plan_launch", plan_fb, dev.stream)
+    dev.sync()
+    plan_fb_wall_s = (time.perf_counter() - t) / reps
+    B.call("bc_amd_plan_destroy", plan_fb)
+
     # parity: NTT bytes identical to the plain calls, MSM folds identical and equal to the closed form
     ntt_ok = all(np.array_equal(dev.download(y_plain[j], (1 << L, 8)), dev.download(y_batch[j], (1 << L, 8))) for j, L in enumerate(ntt_logs))
     f_plain, f_batch = lab.folds(r_plain), lab.folds(r_batch)
     exp = lab.expected_points(d_s, d_k, [1 << L for L in msm_logs])
     msm_ok = bool(np.array_equal(f_plain, f_batch) and np.array_equal(f_batch, exp))
-    for p in [d_b, d_s, x_pool] + d_k + r_plain + r_batch + y_plain + y_batch:
+    fb_ok = bool(np.array_equal(lab.folds(r_fixed), exp))
+    for p in [d_b, d_s, x_pool] + d_k + r_plain + r_batch + y_plain + y_batch + r_fixed + list(tables.values()):
         dev.free(p)
     jobs = len(msm_logs) + len(ntt_logs)
     return {
@@ -388,14 +433,18 @@ def batched(lab, reps):
         "jobs": jobs,
         "one_at_a_time_jobs_per_s": round(jobs / plain_s, 1),
         "one_at_a_time_ms": round(plain_s * 1e3, 2),
-        "batch_calls_ms": round(batch_ms, 2),
-        "plan_ms": round(plan_ms, 2),
-        "plan_wall_ms": round(plan_wall_s * 1e3, 2),
-        "plan_jobs_per_s": round(jobs / plan_wall_s, 1),
-        "speedup_vs_one_at_a_time": round(plain_s / plan_wall_s, 2),
+        "plan_wall_ms": round(plan_fb_wall_s * 1e3, 2),
+        "plan_jobs_per_s": round(jobs / plan_fb_wall_s, 1),
+        "speedup_vs_one_at_a_time": round(plain_s / plan_fb_wall_s, 2),
+        "fixed_base": {"widths": list(FIXED_BASE_WIDTHS), "tables_s": round(table_s, 3), "tables_gib": round(table_bytes / 2**30, 2),
+                       "note": "the plan's MSMs run over fixed-base tables of the resident bases (built once, outside the timed region)"},
+        "bases_jobs": {"batch_calls_ms": round(batch_ms, 2), "plan_ms": round(plan_ms, 2), "plan_wall_ms": round(plan_wall_s * 1e3, 2),
+                       "plan_jobs_per_s": round(jobs / plan_wall_s, 1), "speedup_vs_one_at_a_time": round(plain_s / plan_wall_s, 2),
+                       "note": "the same plan with the MSMs over the bases themselves"},
         "plan_create_s": round(plan_create_s, 2),
         "plan_workspace_gib": round(ws.value / 2**30, 2),
-        "parity": {"ntt_bytes_equal_plain": bool(ntt_ok), "msm_folds_equal_plain_and_closed_form": msm_ok},
+        "parity": {"ntt_bytes_equal_plain": bool(ntt_ok), "msm_folds_equal_plain_and_closed_form": msm_ok,
+                   "fixed_base_msm_folds_equal_closed_form": fb_ok},
     }
 
 
@@ -437,12 +486,16 @@ class ServedJobs:
         else:
             pn_call(self.lab, prm[0], self.pn_ks, out, prm[1], prm[2])
 
-    def job(self, j):
+    def job(self, j, tables=None):
         kind, prm, out = self.jobs[j]
         job = B.bc_amd_job()
         if kind == "msm":
             job.kind = B.BC_AMD_JOB_MSM
-            job.msm = B.msm_job(self.d_b, prm[1], out, prm[0])
+            if tables:
+                c = fixed_base_width(prm[0])
+                job.msm = B.msm_job(tables[c], prm[1], out, prm[0], c, 0, 22)
+            else:
+                job.msm = B.msm_job(self.d_b, prm[1], out, prm[0])
         elif kind == "ntt":
             job.kind = B.BC_AMD_JOB_NTT
             job.ntt = B.ntt_job(prm[1], out, prm[0])
@@ -485,12 +538,12 @@ class ServedJobs:
             dev.free(p)
 
 
-def run_served(srv, sj, arrivals=None):
+def run_served(srv, sj, arrivals=None, tables=None):
     """Submits every job (at `arrivals` seconds from start, or at once); returns stats per job."""
     n = len(sj.jobs)
     tickets = [None] * n
     stats = [None] * n
-    jobs = [sj.job(j) for j in range(n)]
+    jobs = [sj.job(j, tables) for j in range(n)]
 
     def waiter():
         for j in range(n):
@@ -530,11 +583,19 @@ def served(lab, n_jobs, budget_gib, slots):
     fp_plain = sj.fingerprints()
     closed_ok = sj.msm_closed_form_ok(fp_plain)
 
+    tables, table_s, table_bytes = fixed_base_tables(lab, sj.d_b, 22)
     srv = B.Server(hbm_budget_bytes=int(budget_gib * 2**30), max_in_flight=slots)
     try:
+        # capacity with the MSMs over the bases themselves
         run_served(srv, sj)  # warm-up (pool growth)
         sj.clear()
         st = run_served(srv, sj)
+        cap_plain_jobs = n / ((max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9)
+        mismatches_plain_jobs = [j for j, f in sj.fingerprints().items() if f != fp_plain[j]]
+        # the served form: MSMs over fixed-base tables of the resident bases
+        run_served(srv, sj, tables=tables)
+        sj.clear()
+        st = run_served(srv, sj, tables=tables)
         cap_s = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
         t0 = min(s.submit_ns for s in st)
         batches = sorted({(s.start_ns, s.enqueued_ns, s.done_ns, s.batch_jobs, s.slot) for s in st})
@@ -551,7 +612,7 @@ def served(lab, n_jobs, budget_gib, slots):
             arrivals = np.cumsum(rng.exponential(1.0 / lam, n))
             if frac == 0.7:
                 trace07 = arrivals
-            st = run_served(srv, sj, arrivals)
+            st = run_served(srv, sj, arrivals, tables=tables)
             lat = [(s.done_ns - s.submit_ns) / 1e6 for s in st]
             span = (max(s.done_ns for s in st) - min(s.submit_ns for s in st)) / 1e9
             loads["%.1f" % frac] = {"rate_jobs_per_s": round(lam, 1), "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
@@ -570,6 +631,8 @@ def served(lab, n_jobs, budget_gib, slots):
         dev.sync()
         lat_plain.append((time.perf_counter() - t0 - trace07[j]) * 1e3)
     sj.free()
+    for p in tables.values():
+        dev.free(p)
     return {
         "workload": "Poisson mix: 50% MSM 2^14..2^22, 35% NTT 2^12..2^22, 15% PN 4x2^20 (random cycles 1..64)",
         "jobs": n,
@@ -577,11 +640,16 @@ def served(lab, n_jobs, budget_gib, slots):
         "one_at_a_time_jobs_per_s": round(n / plain_s, 1),
         "served_capacity_jobs_per_s": round(capacity, 1),
         "speedup_vs_one_at_a_time": round(capacity / (n / plain_s), 2),
+        "fixed_base": {"widths": list(FIXED_BASE_WIDTHS), "tables_s": round(table_s, 3), "tables_gib": round(table_bytes / 2**30, 2),
+                       "note": "served MSMs run over fixed-base tables of the resident 2^22 bases (built once)"},
+        "bases_jobs": {"served_capacity_jobs_per_s": round(cap_plain_jobs, 1), "speedup_vs_one_at_a_time": round(cap_plain_jobs / (n / plain_s), 2),
+                       "note": "capacity with the MSMs over the bases themselves"},
         "capacity_batches_ms": timeline,
         "capacity_submit_span_ms": round(submit_span, 2),
         "latency_under_poisson": loads,
         "one_at_a_time_fifo_at_0.7": {"p50_ms": round(pct(lat_plain, 50), 2), "p99_ms": round(pct(lat_plain, 99), 2)},
-        "parity": {"served_equal_plain": not mismatches, "mismatched_jobs": mismatches[:8], "msm_closed_form": bool(closed_ok)},
+        "parity": {"served_equal_plain": not mismatches and not mismatches_plain_jobs, "mismatched_jobs": (mismatches + mismatches_plain_jobs)[:8],
+                   "msm_closed_form": bool(closed_ok)},
     }
 
 

@@ -141,6 +141,7 @@ class bc_amd_msm_job(ctypes.Structure):
         ("log_scalars_count", ctypes.c_uint),
         ("window_bits", ctypes.c_uint),
         ("piece", ctypes.c_uint),
+        ("fixed_base_log_count", ctypes.c_uint),
     ]
 
 
@@ -240,7 +241,7 @@ HEADER_SYMBOLS = [
 
 # Extension entry points (include/bellman-amd.h) with no counterpart in the reference.
 EXTENSION_SYMBOLS = [
-    "bc_amd_generator_multiples", "bc_amd_msm_fold", "bc_amd_msm_execute_no_alloc", "bc_amd_msm_batch_async", "bc_amd_ntt_batch_async", "bc_amd_msm_get_tuning", "bc_amd_msm_set_tuning",
+    "bc_amd_generator_multiples", "bc_amd_msm_fold", "bc_amd_msm_execute_no_alloc", "bc_amd_msm_fixed_base_table", "bc_amd_msm_batch_async", "bc_amd_ntt_batch_async", "bc_amd_msm_get_tuning", "bc_amd_msm_set_tuning",
     "bc_amd_ntt_get_split", "bc_amd_ntt_set_split", "bc_amd_msm_autotune", "bc_amd_ntt_autotune", "bc_amd_tuning_save",
     "bc_amd_tuning_load", "bc_amd_plan_create", "bc_amd_plan_workspace_bytes", "bc_amd_plan_launch", "bc_amd_plan_destroy",
     "bc_amd_job_footprint", "bc_amd_job_workspace_bytes", "bc_amd_server_create", "bc_amd_server_submit", "bc_amd_server_wait", "bc_amd_server_destroy",
@@ -296,8 +297,8 @@ def ptr(a):
     return ctypes.c_void_p(a) if isinstance(a, int) else a.ctypes.data_as(ctypes.c_void_p)
 
 
-def msm_job(bases, scalars, results, log_n, window_bits=0, piece=0):
-    return bc_amd_msm_job(bases, scalars, results, log_n, window_bits, piece)
+def msm_job(bases, scalars, results, log_n, window_bits=0, piece=0, fixed_base_log_count=0):
+    return bc_amd_msm_job(bases, scalars, results, log_n, window_bits, piece, fixed_base_log_count)
 
 
 def ntt_job(inputs, outputs, log_n, bitrev=False, inverse=False, log_ext=0, coset=0):

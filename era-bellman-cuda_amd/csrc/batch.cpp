@@ -96,8 +96,15 @@ hipError_t resolve_msm(const bc_amd_msm_job &in, msm_job *out) {
   out->log_n = in.log_scalars_count;
   out->c = in.window_bits ? in.window_bits : t.c;
   out->piece = in.piece ? in.piece : t.piece;
+  out->stride = 0;
   if (out->c < 3 || out->c > 24 || out->piece > MSM_MAX_PIECE)
     return hipErrorInvalidValue;
+  if (in.fixed_base_log_count) {
+    // the table was made with a width the job must name, over at least its points
+    if (!in.window_bits || in.fixed_base_log_count > 28 || in.log_scalars_count > in.fixed_base_log_count)
+      return hipErrorInvalidValue;
+    out->stride = 1u << in.fixed_base_log_count;
+  }
   return hipSuccess;
 }
 
@@ -186,9 +193,9 @@ hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const st
 // level-0 window array (152 B), and 128 B per bucket-split thread, with 256 B
 // of rounding per split level.
 size_t msm_footprint(const msm_job &j) {
-  const uint64_t W = (254 - 1) / j.c + 1;
-  const uint64_t E = W << j.log_n, nb = W << (j.c - 1);
-  return (size_t)(28 * E + 16384 * W + 132 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c) + 256 * 8 + (1u << 20));
+  const uint64_t Wd = (254 - 1) / j.c + 1, W = j.stride ? 1 : Wd; // digit and bucket windows
+  const uint64_t E = Wd << j.log_n, nb = W << (j.c - 1);
+  return (size_t)(28 * E + 16384 * W + 132 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c, (unsigned)W) + 256 * 8 + (1u << 20));
 }
 
 // Coset transforms hold one twist table (2^12 fine + 2^(L-12) coarse entries).

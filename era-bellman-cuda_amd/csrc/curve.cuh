@@ -125,6 +125,16 @@ BC_HD g1_jacobian xyzz_to_jacobian(const g1_xyzz &p) {
   return {X, Y, Z};
 }
 
+// affine (x, y) of an XYZZ point with one inversion: t = (ZZ*ZZZ)^-1,
+// 1/ZZ = ZZZ*t, 1/ZZZ = ZZ*t; infinity -> (0, 0); canonical coordinates
+BC_HD g1_affine xyzz_to_affine(const g1_xyzz &p) {
+  if (xyzz_is_infinity(p))
+    return {fe_zero(), fe_zero()};
+  const fe t = bc::inverse<Fq>(mul<Fq>(p.zz, p.zzz));
+  const fe izz = mul<Fq>(p.zzz, t), izzz = mul<Fq>(p.zz, t);
+  return {mul<Fq>(p.x, izz), mul<Fq>(p.y, izzz)};
+}
+
 __device__ __forceinline__ g1_affine load_affine(const g1_affine *p) { return {load_fe(&p->x), load_fe(&p->y)}; }
 
 __device__ __forceinline__ g1_xyzz load_xyzz(const g1_xyzz *p) { return {load_fe(&p->x), load_fe(&p->y), load_fe(&p->zz), load_fe(&p->zzz)}; }

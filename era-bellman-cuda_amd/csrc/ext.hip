@@ -23,16 +23,6 @@ constexpr unsigned COMB_ENTRIES = 256;
 g1_affine *g_comb[MAX_DEVICES];
 std::mutex g_comb_mutex;
 
-// affine (x, y) of an XYZZ point with one inversion: t = (ZZ*ZZZ)^-1,
-// 1/ZZ = ZZZ*t, 1/ZZZ = ZZ*t; infinity -> (0, 0)
-BC_HD g1_affine xyzz_to_affine(const g1_xyzz &p) {
-  if (xyzz_is_infinity(p))
-    return {fe_zero(), fe_zero()};
-  const fe t = bc::inverse<Fq>(mul<Fq>(p.zz, p.zzz));
-  const fe izz = mul<Fq>(p.zzz, t), izzz = mul<Fq>(p.zz, t);
-  return {mul<Fq>(p.x, izz), mul<Fq>(p.y, izzz)};
-}
-
 __global__ __launch_bounds__(128) void comb_table_kernel(const g1_affine *base_per_byte, g1_affine *table) {
   const unsigned id = blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= COMB_BYTES * COMB_ENTRIES)

@@ -29,6 +29,10 @@ struct msm_job {
   unsigned log_n;
   unsigned c;     // window bits, 3..24
   unsigned piece; // sorted (digit, point) entries per accumulating thread, 1..MSM_MAX_PIECE
+  // 0: `bases` are the points.  Otherwise `bases` is a fixed-base table of
+  // ceil(254 / c) blocks of `stride` points, block w holding [2^(c w)] P_i
+  // (msm_fixed_base_table): all digits go into one window of buckets.
+  uint32_t stride = 0;
 };
 
 // Runs up to MSM_MAX_JOBS MSMs with one launch per phase for the whole batch;
@@ -36,7 +40,11 @@ struct msm_job {
 hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar);
 
 // Threads (one XYZZ point of scratch each) over all bucket-split levels of one
-// job with window width c.
-uint64_t msm_split_threads(unsigned c);
+// job with `windows` bucket windows of width c.
+uint64_t msm_split_threads(unsigned c, unsigned windows);
+
+// table[w * 2^log_n + i] = [2^(c w)] bases[i], affine Montgomery, for the
+// ceil(254 / c) windows w; bases are valid points (not the affine (0, 0)).
+hipError_t msm_fixed_base_table(const g1_affine *bases, unsigned log_n, unsigned c, g1_affine *table, hipStream_t s);
 
 } // namespace bc

@@ -101,10 +101,12 @@ struct job_desc {
   uint32_t top_begin;    // first top-digit slot
   uint32_t chunk_begin;  // first accumulation chunk of the job
   uint32_t slot_begin;   // first partial-sum slot of the job
+  uint32_t stride;       // fixed-base table: points per window block (0: plain bases)
   uint16_t piece;        // sorted entries per accumulation chunk
   uint8_t log_n;         // 2^log_n scalars
   uint8_t c;             // window bits
-  uint8_t W;             // windows
+  uint8_t Wd;            // digit windows of a scalar, ceil(254 / c)
+  uint8_t W;             // bucket windows: Wd, or 1 over a fixed-base table
 };
 
 static_assert(MSM_MAX_JOBS <= SEG_MAX_GROUPS, "one sort segment group per job");
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
   const uint32_t per_xcd = gridDim.x >> 3;
   const uint32_t blk = blockIdx.x < (per_xcd << 3) ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
   const job_desc &jd = jobs.job[find_job(jobs, blk, by_digit_block())];
-  const uint32_t k = blk - jd.digit_block, n = 1u << jd.log_n, c = jd.c, W = jd.W;
+  const uint32_t k = blk - jd.digit_block, n = 1u << jd.log_n, c = jd.c, W = jd.W, Wd = jd.Wd;
   const uint32_t top = 1u << (c - 1);
   const unsigned dbits = c < bits0 ? c : bits0; // keys are below 2^c
   const uint32_t nd = 1u << dbits;
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
     const uint32_t global_sign = lt(pos, neg) ? 0u : SIGN;
     const fe kk = global_sign ? neg : pos;
     uint32_t borrow = 0;
-    for (unsigned w = 0; w < W; w++) {
+    for (unsigned w = 0; w < Wd; w++) {
       uint32_t d = extract_bits(kk, w * c, c) + borrow;
       uint32_t sign = global_sign;
       borrow = 0;
@@ -210,7 +212,9 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
         sign ^= SIGN;
       }
       const uint32_t key = d ? d - 1 : top;
-      entries[jd.entry_begin + ((uint64_t)w << jd.log_n) + i] = (uint64_t)(sign | i) << 32 | key; // key low, point reference high
+      // key low, point reference high: point i, or [2^(c w)] point i of a fixed-base table
+      const uint32_t point = jd.stride ? w * jd.stride + i : i;
+      entries[jd.entry_begin + ((uint64_t)w << jd.log_n) + i] = (uint64_t)(sign | point) << 32 | key;
       if (first_table) {
         // zero digits (often most lanes of a wave in the high windows) are
         // counted once per wave instead of once per lane
@@ -242,8 +246,9 @@ __device__ __forceinline__ void bucket_run_ends(const job_desc &jd, uint32_t e, 
   const uint32_t top = 1u << (jd.c - 1);
   if (k == top)
     return; // zero digits
-  const uint32_t local = e - jd.entry_begin, w = local >> jd.log_n;
-  const uint32_t seg_begin = jd.entry_begin + (w << jd.log_n), seg_end = seg_begin + (1u << jd.log_n);
+  // a job over a fixed-base table sorts all its entries as one window
+  const uint32_t local = e - jd.entry_begin, w = jd.stride ? 0u : local >> jd.log_n;
+  const uint32_t seg_begin = jd.entry_begin + (w << jd.log_n), seg_end = seg_begin + ((uint32_t)(jd.stride ? jd.Wd : 1u) << jd.log_n);
   const uint32_t b = jd.bucket_begin + w * top + k;
   if (e == seg_begin || k_prev != k)
     start[b] = e;
@@ -328,7 +333,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   const g1_affine *__restrict__ bases = jd.bases;
   const uint32_t k = t - jd.chunk_begin, top = 1u << (jd.c - 1);
   const uint32_t begin = jd.entry_begin + k * jd.piece;
-  const uint32_t job_end = jd.entry_begin + ((uint32_t)jd.W << jd.log_n);
+  const uint32_t job_end = jd.entry_begin + ((uint32_t)jd.Wd << jd.log_n);
   const uint32_t end = begin + jd.piece < job_end ? begin + jd.piece : job_end;
   g1_xyzz *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
   constexpr uint32_t NONE = 0xffffffffu;
@@ -347,7 +352,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     const g1_affine pt2 = point_of(ent2, pt1);
     const uint32_t key = (uint32_t)ent0;
     if (key != top) { // zero digits add nothing
-      const uint32_t b = jd.bucket_begin + ((e - jd.entry_begin) >> jd.log_n) * top + key;
+      const uint32_t b = jd.bucket_begin + (jd.stride ? 0u : (e - jd.entry_begin) >> jd.log_n) * top + key;
       if (b != cur) {
         if (cur != NONE)
           store_xyzz(slots + cur, acc);
@@ -482,8 +487,10 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_tabl
   const job_desc &jd = jobs.job[find_job(jobs, t, by_bucket())];
   const uint32_t B = 1u << (jd.c - 1);
   const uint32_t local = t - jd.bucket_begin, w = local >> (jd.c - 1), k = local & (B - 1);
-  const uint32_t top_signed = jd.c - (jd.W * jd.c - 254) - 1;
-  const uint32_t top_digit = w == jd.W - 1 ? (1u << top_signed) : B;
+  // the last digit window tops out below B (a fixed-base job's single bucket
+  // window takes every window's digits, up to B)
+  const uint32_t top_signed = jd.c - (jd.Wd * jd.c - 254) - 1;
+  const uint32_t top_digit = w == jd.W - 1 && !jd.stride ? (1u << top_signed) : B;
   // slot k of a window holds bucket k; slot 0 also hands the bucket of digit B
   // to the window's top when that is its top digit, and the top digit's own
   // slot in the last window hands it there instead
@@ -616,11 +623,14 @@ __global__ __launch_bounds__(ACC_BLOCK) void fold_kernel(const fold_table segs, 
 // 6. bit i = w*c + j of a job is its window's top bucket (j == c-1 or i == 253),
 // else bucket 1 of the one-bit window reached by following the splits: a width-s
 // window x holds bits [0, t) in child 2x and [t, s) in child 2x+1, t = ceil(s/2).
+// A job over a fixed-base table has one window: bits [0, c) as above, the
+// bits above are infinity (its fold is the same point).
 struct result_desc {
   const g1_xyzz *A_final;
   const g1_xyzz *tops;
   g1_jacobian *out;
   uint32_t c;
+  uint32_t single; // one bucket window (fixed-base table)
 };
 
 struct result_table {
@@ -634,10 +644,16 @@ __global__ void result_kernel(const result_table jobs) {
     return;
   const result_desc &rd = jobs.job[t / 254];
   const uint32_t i = t % 254, c = rd.c;
+  if (rd.single && i >= c) {
+    store_fe(&rd.out[i].x, fe_zero());
+    store_fe(&rd.out[i].y, fe_one<Fq>());
+    store_fe(&rd.out[i].z, fe_zero());
+    return;
+  }
   const uint32_t w = i / c;
   uint32_t j = i % c;
   g1_xyzz v;
-  if (j == c - 1 || i == 253) {
+  if (j == c - 1 || (i == 253 && !rd.single)) {
     v = load_xyzz(rd.tops + w);
   } else {
     uint32_t x = w, s = c - 1;
@@ -679,9 +695,9 @@ uint32_t split_parts(uint32_t X, unsigned width) {
 
 } // namespace
 
-uint64_t msm_split_threads(unsigned c) {
+uint64_t msm_split_threads(unsigned c, unsigned windows) {
   uint64_t total = 0;
-  uint32_t X = (254 - 1) / c + 1;
+  uint32_t X = windows;
   for (unsigned width = c - 1; width > 1; width = (width + 1) / 2, X *= 2) {
     const unsigned t = (width + 1) / 2, hs = width - t;
     total += ((uint64_t)split_parts(X, width) << (t - hs)) * ((uint64_t)(2 * X) << t);
@@ -711,28 +727,37 @@ hipError_t layout_batch(const msm_job *jobs, unsigned count, batch_layout *out) 
     if (c < 3 || c > 24 || jobs[j].log_n > 28 || jobs[j].piece == 0 || jobs[j].piece > MSM_MAX_PIECE || !jobs[j].bases || !jobs[j].scalars ||
         !jobs[j].results)
       return hipErrorInvalidValue;
+    // a fixed-base table holds ceil(254 / c) blocks of `stride` >= 2^log_n
+    // points, indexed below 2^31 (the entry's top bit is the sign)
+    const uint64_t wd = (254 - 1) / c + 1;
+    if (jobs[j].stride && (jobs[j].stride < (1ull << jobs[j].log_n) || wd * jobs[j].stride >= (1ull << 31)))
+      return hipErrorInvalidValue;
     job_desc &d = jt.job[j];
     const uint32_t n = 1u << jobs[j].log_n;
     d.bases = jobs[j].bases;
     d.scalars = jobs[j].scalars;
     d.log_n = (uint8_t)jobs[j].log_n;
     d.c = (uint8_t)c;
-    d.W = (uint8_t)((254 - 1) / c + 1);
+    d.Wd = (uint8_t)wd;
+    d.W = jobs[j].stride ? 1 : d.Wd;
+    d.stride = jobs[j].stride;
     d.piece = (uint16_t)jobs[j].piece;
-    if (entries + (uint64_t)d.W * n >= (1ull << 31))
+    if (entries + (uint64_t)d.Wd * n >= (1ull << 31))
       return hipErrorInvalidValue;
     d.entry_begin = (uint32_t)entries;
     d.bucket_begin = (uint32_t)nb64;
     d.top_begin = tops_total;
     d.chunk_begin = (uint32_t)chunks;
     d.slot_begin = (uint32_t)slots;
-    const uint64_t job_chunks = ((uint64_t)d.W * n + d.piece - 1) / d.piece, job_buckets = (uint64_t)d.W << (c - 1);
-    entries += (uint64_t)d.W * n;
+    const uint64_t job_chunks = ((uint64_t)d.Wd * n + d.piece - 1) / d.piece, job_buckets = (uint64_t)d.W << (c - 1);
+    entries += (uint64_t)d.Wd * n;
     nb64 += job_buckets;
     tops_total += d.W;
     chunks += job_chunks;
     slots += job_buckets + job_chunks; // slot(b, k) <= (buckets - 1) + (chunks - 1)
-    max_np = std::max<uint32_t>(max_np, (n + d.piece - 1) / d.piece + 1); // a bucket holds at most n entries
+    // a bucket holds at most n entries, or Wd n in a fixed-base job's single window
+    const uint64_t bucket_max = (uint64_t)(d.stride ? d.Wd : 1u) * n;
+    max_np = std::max<uint32_t>(max_np, (uint32_t)((bucket_max + d.piece - 1) / d.piece + 1));
   }
   if (entries >= (1ull << 31) || nb64 >= (1ull << 31) || slots >= (1ull << 32))
     return hipErrorInvalidValue;
@@ -766,9 +791,13 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
     BC_TRY(ar.get(&digits, entries));
     seg_group groups[MSM_MAX_JOBS];
     unsigned key_bits = 0; // local keys are <= 2^(c-1)
+    bool fixed_base = false;
     for (unsigned j = 0; j < count; j++) {
-      groups[j] = {jt.job[j].entry_begin, 1u << jt.job[j].log_n, jt.job[j].W};
-      key_bits = std::max<unsigned>(key_bits, jt.job[j].c);
+      // a sort segment per window, or one over all of a fixed-base job's entries
+      const job_desc &d = jt.job[j];
+      groups[j] = d.stride ? seg_group{d.entry_begin, (uint32_t)d.Wd << d.log_n, 1u} : seg_group{d.entry_begin, 1u << d.log_n, d.W};
+      key_bits = std::max<unsigned>(key_bits, d.c);
+      fixed_base |= d.stride != 0;
     }
     // the digits kernel counts the sort's first digit when its per-window
     // counters fit DIGITS_MAX_LDS; otherwise the sort counts them itself
@@ -778,7 +807,8 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
     for (unsigned j = 0; j < count; j++)
       lds = std::max(lds, sizeof(uint32_t) * ((((size_t)jt.job[j].W << std::min<unsigned>(jt.job[j].c, fp.bits)) + 1) / 2));
     uint32_t *first_table = nullptr;
-    if (fp.bits > 0 && lds <= DIGITS_MAX_LDS && fp.table_entries < (1ull << 32)) {
+    // (the digits kernel counts per window tile: not for a fixed-base job's single segment)
+    if (fp.bits > 0 && lds <= DIGITS_MAX_LDS && fp.table_entries < (1ull << 32) && !fixed_base) {
       BC_TRY(ar.get(&first_table, fp.table_entries));
       for (unsigned j = 0; j < count; j++)
         jt.job[j].table_begin = (uint32_t)fp.table_begin[j];
@@ -912,7 +942,7 @@ hipError_t reduce_phase(const batch_layout &bl, const msm_job *jobs, unsigned co
   result_table rt{};
   rt.count = count;
   for (unsigned j = 0; j < count; j++)
-    rt.job[j] = {cur[j], tops + jt.job[j].top_begin, jobs[j].results, jt.job[j].c};
+    rt.job[j] = {cur[j], tops + jt.job[j].top_begin, jobs[j].results, jt.job[j].c, jt.job[j].stride ? 1u : 0u};
   result_kernel<<<grid_for(254 * count, 256), 256, 0, s>>>(rt);
   BC_LAUNCH_CHECK();
   for (g1_xyzz *p : level_buffers)
@@ -921,6 +951,39 @@ hipError_t reduce_phase(const batch_layout &bl, const msm_job *jobs, unsigned co
 }
 
 } // namespace
+
+namespace {
+
+// Fixed-base table: thread i writes [2^(c w)] P_i for every window w, by c
+// doublings from the window before and one inversion per entry.
+__global__ __launch_bounds__(MSM_BLOCK) void fixed_base_kernel(const g1_affine *bases, uint32_t n, unsigned c, unsigned windows, g1_affine *table) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n)
+    return;
+  const g1_affine p = load_affine(bases + i);
+  store_fe(&table[i].x, p.x);
+  store_fe(&table[i].y, p.y);
+  g1_xyzz acc = xyzz_from_affine(p);
+  for (unsigned w = 1; w < windows; w++) {
+    for (unsigned k = 0; k < c; k++)
+      acc = xyzz_dbl(acc);
+    const g1_affine a = xyzz_to_affine(acc);
+    g1_affine *dst = table + (size_t)w * n + i;
+    store_fe(&dst->x, a.x);
+    store_fe(&dst->y, a.y);
+  }
+}
+
+} // namespace
+
+hipError_t msm_fixed_base_table(const g1_affine *bases, unsigned log_n, unsigned c, g1_affine *table, hipStream_t s) {
+  const uint64_t windows = (254 - 1) / (c ? c : 1) + 1;
+  if (!bases || !table || c < 3 || c > 24 || log_n > 28 || (windows << log_n) >= (1ull << 31))
+    return hipErrorInvalidValue;
+  const uint32_t n = 1u << log_n;
+  fixed_base_kernel<<<grid_for(n, MSM_BLOCK), MSM_BLOCK, 0, s>>>(bases, n, c, (unsigned)windows, table);
+  return hipGetLastError();
+}
 
 hipError_t msm_batch_device(const msm_job *jobs, unsigned count, arena &ar) {
   if (count == 0)
@@ -1163,6 +1226,10 @@ bc_error msm_set_up() {
 }
 
 bc_error bc_amd_msm_execute_no_alloc(msm_configuration configuration) { return as_bc(msm_execute_no_alloc(configuration)); }
+
+bc_error bc_amd_msm_fixed_base_table(const void *bases, unsigned log_count, unsigned window_bits, void *table, bc_stream stream) {
+  return as_bc(msm_fixed_base_table(static_cast<const g1_affine *>(bases), log_count, window_bits, static_cast<g1_affine *>(table), to_stream(stream)));
+}
 
 bc_error msm_execute_async(msm_configuration configuration) { return as_bc(msm_execute(configuration)); }
 

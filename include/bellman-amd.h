@@ -57,7 +57,23 @@ typedef struct {
   unsigned log_scalars_count;
   unsigned window_bits; /* 0: the tuned width for the size; else 3..24 */
   unsigned piece;       /* 0: tuned; else 1..65535 sorted (digit, point) entries per accumulating thread */
+  /* 0: bases are the points.  k >= 1: bases is a fixed-base table made by
+   * bc_amd_msm_fixed_base_table over 2^k >= 2^log_scalars_count points with
+   * this job's window_bits (which must then be set).  Every digit of every
+   * window then goes into one window of buckets: per-bit results 0..c-1, the
+   * ones above are infinity, and the folded point is the same. */
+  unsigned fixed_base_log_count;
 } bc_amd_msm_job;
+
+/* Fixed-base table for MSMs over bases that stay resident: table[w * 2^k + i]
+ * = [2^(c w)] bases[i] (affine Montgomery, canonical) for the ceil(254 / c)
+ * windows w of width c = window_bits (3..24), k = log_count, (254/c rounded up)
+ * * 2^k < 2^31.  The table takes ceil(254 / c) * 2^k * 64 bytes of device
+ * memory.  An MSM over it does the same number of bucket additions as over the
+ * bases but reduces one window of buckets instead of ceil(254 / c), so wider
+ * windows (fewer additions) pay off.  Cost: c doublings and one inversion per
+ * entry, once. */
+bc_error bc_amd_msm_fixed_base_table(const void *bases, unsigned log_count, unsigned window_bits, void *table, bc_stream stream);
 
 /* One transform with the ntt_configuration contract. */
 typedef struct {
