@@ -21,14 +21,14 @@
 //                  of the batch, each window a segment (c key bits: two passes
 //                  of <= 11 bits up to c = 22; per-tile LDS histograms,
 //                  wave-ballot ranking, no global atomics, so heavy buckets
-//                  cost nothing extra), then run detection gives each bucket's
-//                  [start, end);
+//                  cost nothing extra);
 //   3. accumulate: the sorted entries of a job are cut into chunks of
 //                  `piece` entries, one per thread; a thread adds its chunk's
 //                  points with XYZZ mixed additions into one running sum per
 //                  bucket it meets, so every lane does the same work and a
 //                  heavy bucket (e.g. digit 1 of window 0 when many scalars
-//                  are 1) is spread over many threads;
+//                  are 1) is spread over many threads; the same pass records
+//                  each bucket's run [start, end) of sorted entries;
 //   4. fix-up:     a bucket's sums lie in consecutive slots; buckets of more
 //                  than 32 are listed and reduced by one-wave blocks (chunks
 //                  of 512 sums: serial sums of 8, then an LDS tree); smaller
@@ -237,61 +237,6 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
   }
 }
 
-// 2. after the sort: bucket b occupies [start[b], end[b]) of the sorted entries
-// (window w of a job is the segment [entry_begin + w n, entry_begin + (w+1) n))
-constexpr unsigned RUN_ITEMS = 8; // entries per thread of bucket_runs_kernel
-
-__device__ __forceinline__ void bucket_run_ends(const job_desc &jd, uint32_t e, uint32_t k, uint32_t k_prev, uint32_t k_next, uint32_t *start,
-                                                uint32_t *end) {
-  const uint32_t top = 1u << (jd.c - 1);
-  if (k == top)
-    return; // zero digits
-  // a job over a fixed-base table sorts all its entries as one window
-  const uint32_t local = e - jd.entry_begin, w = jd.stride ? 0u : local >> jd.log_n;
-  const uint32_t seg_begin = jd.entry_begin + (w << jd.log_n), seg_end = seg_begin + ((uint32_t)(jd.stride ? jd.Wd : 1u) << jd.log_n);
-  const uint32_t b = jd.bucket_begin + w * top + k;
-  if (e == seg_begin || k_prev != k)
-    start[b] = e;
-  if (e + 1 == seg_end || k_next != k)
-    end[b] = e + 1;
-}
-
-// A block covers MSM_BLOCK * RUN_ITEMS consecutive entries.  All keys and
-// their neighbours' keys are loaded first (clamped indexes stay in bounds), so
-// a wave waits for memory once; a block inside one job reads that job's
-// descriptor with scalar loads, a block across jobs looks each entry's job up.
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t entries,
-                                                                uint32_t *__restrict__ start, uint32_t *__restrict__ end) {
-  const uint32_t first = blockIdx.x * (MSM_BLOCK * RUN_ITEMS);
-  const uint32_t last = min(first + MSM_BLOCK * RUN_ITEMS, entries) - 1;
-  const uint32_t *keys = reinterpret_cast<const uint32_t *>(sorted); // the low words (little-endian)
-  uint32_t k[RUN_ITEMS], k_prev[RUN_ITEMS], k_next[RUN_ITEMS];
-#pragma unroll
-  for (unsigned s = 0; s < RUN_ITEMS; s++) {
-    const uint32_t e = min(first + s * MSM_BLOCK + threadIdx.x, last);
-    k[s] = keys[2 * (uint64_t)e];
-    k_prev[s] = keys[2 * (uint64_t)(e ? e - 1 : 0)];
-    k_next[s] = keys[2 * (uint64_t)(e + 1 < entries ? e + 1 : e)];
-  }
-  const unsigned j0 = find_job(jobs, first, by_entry()), j1 = find_job(jobs, last, by_entry());
-  if (j0 == j1) {
-    const job_desc &jd = jobs.job[j0];
-#pragma unroll
-    for (unsigned s = 0; s < RUN_ITEMS; s++) {
-      const uint32_t e = first + s * MSM_BLOCK + threadIdx.x;
-      if (e <= last)
-        bucket_run_ends(jd, e, k[s], k_prev[s], k_next[s], start, end);
-    }
-  } else {
-#pragma unroll
-    for (unsigned s = 0; s < RUN_ITEMS; s++) {
-      const uint32_t e = first + s * MSM_BLOCK + threadIdx.x;
-      if (e <= last)
-        bucket_run_ends(jobs.job[find_job(jobs, e, by_entry())], e, k[s], k_prev[s], k_next[s], start, end);
-    }
-  }
-}
-
 // 3. accumulation over equal chunks of the sorted entries.  Chunk k of a job
 // is its entries [k piece, (k+1) piece); one thread adds the chunk's points
 // into a running XYZZ sum and stores the sum at slot(b, k) = slot_begin +
@@ -300,7 +245,10 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_runs_kernel(const job_table 
 // decreases, so every (chunk, bucket) pair has a slot of its own and a
 // bucket's sums occupy consecutive slots.  All threads of a wave do the same
 // number of additions whatever the bucket sizes (cutting buckets into pieces
-// instead leaves a wave waiting for its longest piece).
+// instead leaves a wave waiting for its longest piece).  The thread that holds
+// a bucket's first (last) sorted entry also records where the bucket's run
+// starts (ends), from the keys before and after it, so no separate pass over
+// the sorted entries is needed.
 
 // piece_offset[b], piece_count[b]: the slots of bucket b's sums, from its run
 // [start, end) of sorted entries (empty buckets have none).  Buckets of more
@@ -325,7 +273,8 @@ __global__ void bucket_slots_kernel(const job_table jobs, const uint32_t *start,
 }
 
 __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t chunks,
-                                                               g1_xyzz *__restrict__ partials) {
+                                                               g1_xyzz *__restrict__ partials, uint32_t *__restrict__ run_start,
+                                                               uint32_t *__restrict__ run_end) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= chunks)
     return;
@@ -335,6 +284,9 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   const uint32_t begin = jd.entry_begin + k * jd.piece;
   const uint32_t job_end = jd.entry_begin + ((uint32_t)jd.Wd << jd.log_n);
   const uint32_t end = begin + jd.piece < job_end ? begin + jd.piece : job_end;
+  // sort segments: every window, or all of a fixed-base job's entries;
+  // (e - entry_begin) & seg_mask is 0 exactly at a segment's first entry
+  const uint32_t seg_mask = jd.stride ? 0xffffffffu : (1u << jd.log_n) - 1;
   g1_xyzz *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
   constexpr uint32_t NONE = 0xffffffffu;
   uint32_t cur = NONE;
@@ -346,6 +298,10 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     return (uint32_t)ent != top ? load_affine(bases + ((uint32_t)(ent >> 32) & ~SIGN)) : keep;
   };
   uint64_t ent0 = sorted[begin], ent1 = sorted[min(begin + 1, last)], ent2 = sorted[min(begin + 2, last)];
+  // the keys around the chunk, for the runs of the buckets it shares with its
+  // neighbours (not read at a job's ends, where a segment ends anyway)
+  uint32_t key_prev = begin != jd.entry_begin ? (uint32_t)sorted[begin - 1] : NONE;
+  const uint32_t key_after = end != job_end ? (uint32_t)sorted[end] : NONE;
   g1_affine pt0 = point_of(ent0, g1_affine{}), pt1 = point_of(ent1, pt0);
   for (uint32_t e = begin; e < end; e++) {
     const uint64_t ent3 = sorted[min(e + 3, last)];
@@ -353,6 +309,12 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     const uint32_t key = (uint32_t)ent0;
     if (key != top) { // zero digits add nothing
       const uint32_t b = jd.bucket_begin + (jd.stride ? 0u : (e - jd.entry_begin) >> jd.log_n) * top + key;
+      // bucket b occupies [run_start[b], run_end[b]) of the sorted entries
+      const uint32_t key_next = e != last ? (uint32_t)ent1 : key_after;
+      if (((e - jd.entry_begin) & seg_mask) == 0 || key_prev != key)
+        run_start[b] = e;
+      if (((e + 1 - jd.entry_begin) & seg_mask) == 0 || key_next != key)
+        run_end[b] = e + 1;
       if (b != cur) {
         if (cur != NONE)
           store_xyzz(slots + cur, acc);
@@ -364,6 +326,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
         pt.y = neg2<Fq>(pt.y);
       acc = xyzz_add_affine(acc, pt);
     }
+    key_prev = key;
     ent0 = ent1;
     ent1 = ent2;
     ent2 = ent3;
@@ -825,13 +788,11 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
     if (first_table)
       BC_TRY(ar.put(first_table));
     BC_TRY(ar.put(digits));
-    BC_TRY(fill_u32(offsets, nb, 0, s));
-    BC_TRY(fill_u32(counts, nb, 0, s));
-    bucket_runs_kernel<<<grid_for(entries, MSM_BLOCK * RUN_ITEMS), MSM_BLOCK, 0, s>>>(jt, sorted, (uint32_t)entries, offsets, counts);
-    BC_LAUNCH_CHECK();
   }
 
-  // 3. accumulation over chunks of the sorted entries; meta = {heavy count, heavy chunk total}
+  // 3. accumulation over chunks of the sorted entries, which also records each
+  // bucket's run [offsets[b], counts[b]) (empty buckets keep [0, 0));
+  // meta = {heavy count, heavy chunk total}
   uint32_t *piece_count = nullptr, *piece_offset = nullptr, *meta = nullptr, *heavy = nullptr;
   g1_xyzz *partials = nullptr;
   BC_TRY(ar.get(&piece_count, nb));
@@ -839,10 +800,12 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
   BC_TRY(ar.get(&meta, 2));
   BC_TRY(ar.get(&partials, slots));
   BC_TRY(ar.get(&heavy, nb));
+  BC_TRY(fill_u32(offsets, nb, 0, s));
+  BC_TRY(fill_u32(counts, nb, 0, s));
   BC_TRY(fill_u32(meta, 2, 0, s));
-  bucket_slots_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_offset, piece_count, nb, heavy, meta);
+  accumulate_kernel<<<grid_for(chunks, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, (uint32_t)chunks, partials, offsets, counts);
   BC_LAUNCH_CHECK();
-  accumulate_kernel<<<grid_for(chunks, ACC_BLOCK), ACC_BLOCK, 0, s>>>(jt, sorted, (uint32_t)chunks, partials);
+  bucket_slots_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, offsets, counts, piece_offset, piece_count, nb, heavy, meta);
   BC_LAUNCH_CHECK();
 
   // 4. buckets of more than SMALL_PIECES sums; smaller ones are summed by the gather
