@@ -1,22 +1,25 @@
 // XYZZ additions on the 29-bit-limb field arithmetic (field29.cuh):
-// add-2008-s (12M + 2S), the EFD formula src/ec.cuh:494-546 also uses.  Sums
-// live in registers in the 2^261 radix; terms arrive in the library's 2^256
-// format and sums leave in it.
+// madd-2008-s (bucket += affine point) and add-2008-s (12M + 2S), the EFD
+// formulas src/ec.cuh:416-546 also uses.  Sums live in registers in the 2^261
+// radix; terms arrive in the library's 2^256 format and sums leave in it.
 //
-// Where they are used (measured on MI355X, same box, 2^24 MSM at c = 20):
-// the bucket-reduction split kernel runs 1.78 -> 1.58 ms on them, and the
-// heavy-bucket chunk sums 0.135 -> 0.123 ms.  The bucket accumulation stays on
-// 32-bit limbs: madd-2008-s on 29-bit limbs was not faster (13.2-13.8 ms either
-// way): its 32-bit-limb version issues fewer 64-bit-result instructions
-// (v_mad_u64_u32, 64-bit shifts), which take twice the issue time of the
-// 32-bit adds the 29-bit limbs save, and its conversions and register use
-// (195 VGPRs, two waves per SIMD) cancel the rest.  The small-bucket gather
-// was slower on them (0.84 -> 0.95 ms): most of its buckets add one or two
-// terms, so the conversions dominate.
+// On the device the products are the generated assembly of mont29_asm.h, and
+// independent products go in pairs (mul29x2, sqr29x2) whose column chains
+// interleave.  Measured on MI355X: a register-only chain of bucket += point
+// runs at 17.8 G additions/s against 14.5 G for the 32-bit-limb madd; in the
+// 2^24 accumulation kernel (random 64-byte point loads, two waves per SIMD at
+// 217 VGPRs, and a clock that drops from about 2.0 to 1.9 GHz under the denser
+// multiplies) the gain is 2-4 % at c = 20 and 7 % at c = 17; the bucket-
+// reduction split kernel runs 3-4 % faster than on the compiler's 29-bit
+// products.  The small-bucket gather stays on 32-bit limbs (0.84 -> 0.95 ms
+// on 29-bit limbs: most of its buckets add one or two terms, so the
+// conversions dominate).
 //
 // Bounds kept (m = p, all limbs normalized unless stated):
-//   X1 < 1.01 m (reduced after every addition), Y1, ZZ1, ZZZ1 < 2^257.3 (products);
-//   term coordinates < 2m in the 2^256 radix, so below 2^259.6 here.
+//   X1 < 2 m (reduced after every addition), Y1, ZZ1, ZZZ1 < 2^257.3 (products);
+//   term coordinates < 2m in the 2^256 radix, so below 2^259.6 here; an affine
+//   point's canonical coordinates become 32 x value < 2^259, and -y is taken
+//   as 64 m - 32 y (lower limbs below 2^31: the wide operand of a product).
 #pragma once
 
 #include "curve.cuh"
@@ -42,10 +45,9 @@ BC_HD g1_xyzz xyzz29_to_xyzz(const xyzz29 &p) { return {to_ext<Fq>(p.x), to_ext<
 BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
   using P = P29<Fq>;
   const f29 X2 = ext_to29(qe.x), Y2 = ext_to29(qe.y), ZZ2 = ext_to29(qe.zz), ZZZ2 = ext_to29(qe.zzz); // < 2^259.6
-  const f29 U1 = mul29<Fq>(acc.x, ZZ2);
-  const f29 U2 = mul29<Fq>(X2, acc.zz);
-  const f29 S1 = mul29<Fq>(acc.y, ZZZ2);
-  const f29 S2 = mul29<Fq>(Y2, acc.zzz);
+  f29 U1, U2, S1, S2;
+  mul29x2<Fq>(acc.x, ZZ2, X2, acc.zz, U1, U2);
+  mul29x2<Fq>(acc.y, ZZZ2, Y2, acc.zzz, S1, S2);
   const f29 Pd = norm29(sub29(U2, U1, P::c16)); // U2 + 16m - U1
   const f29 R = norm29(sub29(S2, S1, P::c16));  // S2 + 16m - S1
   if (is_zero29<Fq>(Pd)) {
@@ -57,19 +59,60 @@ BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
     }
     return;
   }
-  const f29 PP = sqr29<Fq>(Pd);
-  const f29 PPP = mul29<Fq>(Pd, PP);
-  const f29 Q = mul29<Fq>(U1, PP);
-  const f29 RR = sqr29<Fq>(R);
+  f29 PP, RR, PPP, Q, T, TT, ZZ3, ZZZ3;
+  sqr29x2<Fq>(Pd, R, PP, RR);
+  mul29x2<Fq>(Pd, PP, U1, PP, PPP, Q);
   const f29 W = add29(PPP, add29(Q, Q));
   const f29 X3 = reduce_value29<Fq>(norm29(sub29(RR, W, P::c48e2)));
   const f29 QX = norm29(sub29(Q, X3, P::c2));
   const f29 NP = norm29(neg29(PPP, P::c16));
   const f29 Y3 = mul2_29<Fq>(R, QX, S1, NP); // R (Q - X3) - S1 PPP
-  acc.zz = mul29<Fq>(acc.zz, mul29<Fq>(ZZ2, PP));
-  acc.zzz = mul29<Fq>(acc.zzz, mul29<Fq>(ZZZ2, PPP));
-  acc.x = X3;
-  acc.y = Y3;
+  mul29x2<Fq>(ZZ2, PP, ZZZ2, PPP, T, TT);
+  mul29x2<Fq>(acc.zz, T, acc.zzz, TT, ZZ3, ZZZ3);
+  acc = {X3, Y3, ZZ3, ZZZ3};
+}
+
+// acc += (x, y) or (x, -y) (madd-2008-s, 8M + 2S with the lazy Y3 as one
+// product pair), for an affine point with canonical coordinates.  acc may be
+// infinity (`empty`); the point is not.  In the 2^261 radix the point's
+// coordinates are 32 x their 2^256-radix values, below 2^259.
+BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool negate) {
+  using P = P29<Fq>;
+  const f29 X2 = ext_to29(q.x);
+  const f29 Y2p = ext_to29(q.y);
+  // 64 m - y: lower limbs in (2^30 - 2^29, 2^30 + 2^29), a valid wide operand of mul29
+  const f29 Y2 = negate ? neg29(Y2p, P::c64) : Y2p;
+  if (empty) {
+    acc = {reduce_value29<Fq>(X2), reduce_value29<Fq>(norm29(Y2)), one29<Fq>(), one29<Fq>()};
+    empty = false;
+    return;
+  }
+  f29 U2, S2;
+  mul29x2<Fq>(X2, acc.zz, acc.zzz, Y2, U2, S2);   // X2 ZZ1, ZZZ1 Y2 (Y2 the wide operand)
+  const f29 Pd = norm29(sub29(U2, acc.x, P::c2)); // U2 + 2m - X1
+  const f29 R = norm29(sub29(S2, acc.y, P::c16)); // S2 + 16m - Y1
+  if (is_zero29<Fq>(Pd)) {
+    if (is_zero29<Fq>(R)) {
+      // acc equals the point: twice the point by the formulas of the 2^256 radix
+      g1_affine qs = q;
+      if (negate)
+        qs.y = neg2<Fq>(q.y);
+      acc = xyzz29_from_xyzz(xyzz_dbl_affine(qs));
+    } else {
+      empty = true; // acc = -point
+    }
+    return;
+  }
+  f29 PP, RR, PPP, Q, ZZ3, ZZZ3;
+  sqr29x2<Fq>(Pd, R, PP, RR);
+  mul29x2<Fq>(Pd, PP, acc.x, PP, PPP, Q);
+  const f29 W = add29(PPP, add29(Q, Q));
+  const f29 X3 = reduce_value29<Fq>(norm29(sub29(RR, W, P::c48e2)));
+  const f29 QX = norm29(sub29(Q, X3, P::c2));
+  const f29 NP = norm29(neg29(PPP, P::c16));
+  const f29 Y3 = mul2_29<Fq>(R, QX, acc.y, NP); // R (Q - X3) - Y1 PPP
+  mul29x2<Fq>(acc.zz, PP, acc.zzz, PPP, ZZ3, ZZZ3);
+  acc = {X3, Y3, ZZ3, ZZZ3};
 }
 
 // acc += q where either may be infinity (`empty`, or q's ZZ = 0 mod p)

@@ -122,6 +122,7 @@ template <class F> struct P29 {
   static constexpr c29::limbs c2 = c29::borrowed(c29::times(n, 2), 0);
   static constexpr c29::limbs c16 = c29::borrowed(c29::times(n, 16), 0);
   static constexpr c29::limbs c48e2 = c29::borrowed(c29::times(n, 48), 2);
+  static constexpr c29::limbs c64 = c29::borrowed(c29::times(n, 64), 1);
   // floor(x8 * div_magic / 2^52) <= floor(x8 / (n8 + 1)) for x8 < 2^31
   static constexpr uint64_t div_magic = (1ull << 52) / (n.v[8] + 1);
 };
@@ -265,9 +266,18 @@ template <class F> BC_HD bool is_zero29(const f29 &a) {
 // ---------------------------------------------------------------- products
 // Product scanning: column k of a*b + q*m accumulates in one 64-bit value; the
 // first nine columns each fix one reduction digit q_k (making the column's low
-// 29 bits zero), the last eight emit the result limbs.
+// 29 bits zero), the last eight emit the result limbs.  Device code runs the
+// same steps as generated inline assembly (mont29_asm.h: one strict multiply
+// chain per column); the C++ below is the host version and the definition.
+
+#if defined(__HIP_DEVICE_COMPILE__)
+#include "mont29_asm.h"
+#endif
 
 template <class F> BC_HD f29 mul29(const f29 &a, const f29 &b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return mul29_device<F>(a, b);
+#else
   constexpr c29::limbs N = P29<F>::n;
   uint64_t acc = 0;
   uint32_t q[9];
@@ -297,10 +307,14 @@ template <class F> BC_HD f29 mul29(const f29 &a, const f29 &b) {
   }
   t.v[8] = (uint32_t)acc;
   return t;
+#endif
 }
 
 // a^2: cross products once, against the doubled operand (limbs below 2^30)
 template <class F> BC_HD f29 sqr29(const f29 &a) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return sqr29_device<F>(a);
+#else
   constexpr c29::limbs N = P29<F>::n;
   f29 d;
 #pragma unroll
@@ -335,10 +349,14 @@ template <class F> BC_HD f29 sqr29(const f29 &a) {
   }
   t.v[8] = (uint32_t)acc;
   return t;
+#endif
 }
 
 // (a b + c d) / 2^261 with one reduction; all four inputs normalized
 template <class F> BC_HD f29 mul2_29(const f29 &a, const f29 &b, const f29 &c, const f29 &d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return mul2_29_device<F>(a, b, c, d);
+#else
   constexpr c29::limbs N = P29<F>::n;
   uint64_t acc = 0;
   uint32_t q[9];
@@ -372,6 +390,30 @@ template <class F> BC_HD f29 mul2_29(const f29 &a, const f29 &b, const f29 &c, c
   }
   t.v[8] = (uint32_t)acc;
   return t;
+#endif
+}
+
+// two independent products, r = a b and s = c e (a and e may be the wide
+// operands); on the device their column chains are interleaved, which keeps a
+// wave issuing while a multiply's result is in flight.  r and s must not alias
+// any input.
+template <class F> BC_HD void mul29x2(const f29 &a, const f29 &b, const f29 &c, const f29 &e, f29 &r, f29 &s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  mul29x2_device<F>(a, b, c, e, r, s);
+#else
+  r = mul29<F>(a, b);
+  s = mul29<F>(c, e);
+#endif
+}
+
+// r = a^2 and s = c^2, interleaved on the device as mul29x2
+template <class F> BC_HD void sqr29x2(const f29 &a, const f29 &c, f29 &r, f29 &s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  sqr29x2_device<F>(a, c, r, s);
+#else
+  r = sqr29<F>(a);
+  s = sqr29<F>(c);
+#endif
 }
 
 } // namespace bc

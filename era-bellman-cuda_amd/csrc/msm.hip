@@ -290,7 +290,9 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   g1_xyzz *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
   constexpr uint32_t NONE = 0xffffffffu;
   uint32_t cur = NONE;
-  g1_xyzz acc = xyzz_infinity();
+  // the running sum in the 29-bit-limb form (infinity while `empty`)
+  xyzz29 acc{};
+  bool empty = true;
   // software pipeline: entries three ahead, their points two ahead (a zero
   // digit's point is not loaded); indexes past the chunk repeat its last entry
   const uint32_t last = end - 1;
@@ -317,14 +319,11 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
         run_end[b] = e + 1;
       if (b != cur) {
         if (cur != NONE)
-          store_xyzz(slots + cur, acc);
-        acc = xyzz_infinity();
+          store_xyzz(slots + cur, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
+        empty = true;
         cur = b;
       }
-      g1_affine pt = pt0;
-      if ((uint32_t)(ent0 >> 32) & SIGN)
-        pt.y = neg2<Fq>(pt.y);
-      acc = xyzz_add_affine(acc, pt);
+      xyzz29_add_affine(acc, empty, pt0, ((uint32_t)(ent0 >> 32) & SIGN) != 0);
     }
     key_prev = key;
     ent0 = ent1;
@@ -334,7 +333,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
     pt1 = pt2;
   }
   if (cur != NONE)
-    store_xyzz(slots + cur, acc);
+    store_xyzz(slots + cur, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
 }
 
 // Heavy buckets are reduced in chunks of HEAVY_CHUNK partial sums, one block

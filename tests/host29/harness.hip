@@ -75,19 +75,19 @@ void h29_to_ext(int field, const uint32_t *a, uint32_t *out) {
 }
 
 void h29_constants(int field, uint32_t *out) {
-  // n, one, c2, c16, c48e2 (9 limbs each), inv, pos_inv
+  // n, one, c2, c16, c48e2, c64 (9 limbs each), inv, pos_inv
   auto put = [&](int k, const c29::limbs &l) {
     for (int i = 0; i < 9; i++)
       out[9 * k + i] = l.v[i];
   };
   if (field) {
     using P = P29<FqParams>;
-    put(0, P::n); put(1, P::one); put(2, P::c2); put(3, P::c16); put(4, P::c48e2);
-    out[45] = P::inv; out[46] = P::pos_inv;
+    put(0, P::n); put(1, P::one); put(2, P::c2); put(3, P::c16); put(4, P::c48e2); put(5, P::c64);
+    out[54] = P::inv; out[55] = P::pos_inv;
   } else {
     using P = P29<FrParams>;
-    put(0, P::n); put(1, P::one); put(2, P::c2); put(3, P::c16); put(4, P::c48e2);
-    out[45] = P::inv; out[46] = P::pos_inv;
+    put(0, P::n); put(1, P::one); put(2, P::c2); put(3, P::c16); put(4, P::c48e2); put(5, P::c64);
+    out[54] = P::inv; out[55] = P::pos_inv;
   }
 }
 
@@ -105,6 +105,52 @@ void h29_add_chain(const uint32_t *points, int count, uint32_t *out, int *out_em
     xyzz29_accumulate(acc, empty, q);
   }
   *out_empty = empty;
+  const g1_xyzz r = empty ? xyzz_infinity() : xyzz29_to_xyzz(acc);
+  const fe *c[4] = {&r.x, &r.y, &r.zz, &r.zzz};
+  for (int k = 0; k < 4; k++)
+    for (int i = 0; i < 8; i++)
+      out[8 * k + i] = c[k]->v[i];
+}
+
+// sum of `count` affine points (2 x 8 words each, canonical 2^256 radix),
+// each negated when its flag is set, through xyzz29_add_affine, returned in
+// the 2^256 radix.  *out_bounds = 1 when every intermediate sum kept the
+// bounds the addition relies on: normalized limbs, X below 2m and Y, ZZ, ZZZ
+// below 2^257.3 (checked as below 2^257.3 < 1.2312 * 2^257).
+void h29_madd_chain(const uint32_t *points, const int *negate, int count, uint32_t *out, int *out_empty, int *out_bounds) {
+  using P = P29<FqParams>;
+  xyzz29 acc{};
+  bool empty = true;
+  bool ok = true;
+  auto below = [](const f29 &v, const c29::limbs &bound) { // v < bound, both normalized
+    for (int i = 8; i >= 0; i--)
+      if (v.v[i] != bound.v[i])
+        return v.v[i] < bound.v[i];
+    return false;
+  };
+  auto normalized = [](const f29 &v) {
+    for (int i = 0; i < 8; i++)
+      if (v.v[i] > MASK29)
+        return false;
+    return true;
+  };
+  constexpr c29::limbs two_m = c29::times(P::n, 2);
+  c29::limbs big{}; // 1.2312 * 2^257 > 2^257.3
+  big.v[8] = (uint32_t)(1.2312 * (double)(1u << 25));
+  for (int k = 0; k < count; k++) {
+    g1_affine q;
+    for (int i = 0; i < 8; i++) {
+      q.x.v[i] = points[16 * k + i];
+      q.y.v[i] = points[16 * k + 8 + i];
+    }
+    xyzz29_add_affine(acc, empty, q, negate[k] != 0);
+    if (!empty) {
+      ok = ok && normalized(acc.x) && normalized(acc.y) && normalized(acc.zz) && normalized(acc.zzz);
+      ok = ok && below(acc.x, two_m) && below(acc.y, big) && below(acc.zz, big) && below(acc.zzz, big);
+    }
+  }
+  *out_empty = empty;
+  *out_bounds = ok;
   const g1_xyzz r = empty ? xyzz_infinity() : xyzz29_to_xyzz(acc);
   const fe *c[4] = {&r.x, &r.y, &r.zz, &r.zzz};
   for (int k = 0; k < 4; k++)

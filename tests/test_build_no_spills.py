@@ -72,7 +72,7 @@ def instructions(text):
 
 def asm_templates():
     """(instructions, output names, input names) of every inline-asm statement."""
-    text = open(os.path.join(CSRC, "mont_asm.h")).read()
+    text = "".join(open(os.path.join(CSRC, h)).read() for h in ("mont_asm.h", "mont29_asm.h"))
     out = []
     for m in re.finditer(r'asm\("((?:[^"\\]|\\.)*)"\s*:(.*?):(.*?)\);', text, flags=re.S):
         body = m.group(1).replace("\\n", "\n").replace("\\t", " ")

@@ -74,16 +74,18 @@ def samples(rng, hi_bits, count):
 
 def test_constants(h29):
     for field, m in FIELDS.items():
-        c = np.zeros(47, dtype=np.uint32)
+        c = np.zeros(56, dtype=np.uint32)
         h29.h29_constants(field, p_(c))
-        n, one, c2, c16, c48 = (c[9 * k:9 * k + 9] for k in range(5))
+        n, one, c2, c16, c48, c64 = (c[9 * k:9 * k + 9] for k in range(6))
         assert value(n) == m and normalized(n)
         assert value(one) == (1 << 261) % m
-        for k, cc, e in ((2, c2, 0), (16, c16, 0), (48, c48, 2)):
+        for k, cc, e in ((2, c2, 0), (16, c16, 0), (48, c48, 2), (64, c64, 1)):
             assert value(cc) == k * m
             assert all(int(x) >= (1 << (29 + e)) and int(x) < (1 << (29 + e)) + (1 << 29) for x in cc[:8])
-        assert (int(c[45]) * m) % (1 << 29) == M29  # -m^-1
-        assert (int(c[46]) * m) % (1 << 29) == 1
+        # 64m - y for any y below 32m (a canonical coordinate times 2^5) keeps a positive top limb
+        assert int(c64[8]) >= (32 * m) >> 232
+        assert (int(c[54]) * m) % (1 << 29) == M29  # -m^-1
+        assert (int(c[55]) * m) % (1 << 29) == 1
 
 
 def test_conversions(h29):
@@ -227,6 +229,51 @@ def test_xyzz_sums_against_affine_arithmetic(h29, oracle):
         out = np.zeros(32, dtype=np.uint32)
         empty = ctypes.c_int()
         h29.h29_add_chain(p_(arr), len(chain), p_(out), ctypes.byref(empty))
+        if want is None:
+            assert empty.value == 1
+            continue
+        assert empty.value == 0
+        X, Y, ZZ, ZZZ = (wvalue(out[8 * k:8 * k + 8]) * mont_inv % P for k in range(4))
+        assert ZZ and pow(ZZ, 3, P) == pow(ZZZ, 2, P)
+        assert X * pow(ZZ, -1, P) % P == want[0]
+        assert Y * pow(ZZZ, -1, P) % P == want[1]
+
+
+def test_affine_bucket_sums_against_affine_arithmetic(h29, oracle):
+    """Chains of mixed additions bucket += (x, +-y) (the accumulation step):
+    random points, a repeated point (the doubling path), a point followed by
+    its negation (infinity mid-chain, then a fresh start), and a long chain;
+    the sum equals the affine sum computed with Python integers, and every
+    intermediate sum keeps the bounds the next addition relies on."""
+    rng = random.Random(0xBC2906)
+    s = [rng.randrange(1, R) for _ in range(24)]
+    pts = oracle.gen_bases(s)
+    mont = 1 << 256
+    mont_inv = pow(mont, -1, P)
+
+    def decode(pt):
+        return wvalue(pt[:8]) * mont_inv % P, wvalue(pt[8:16]) * mont_inv % P
+
+    chains = [
+        [(0, 0), (1, 0), (2, 1), (3, 0), (4, 1), (5, 0)],
+        [(6, 0), (6, 0)],                     # doubling
+        [(6, 1), (6, 1), (7, 0)],             # doubling of a negated point
+        [(7, 0), (7, 1), (8, 0), (9, 1)],     # infinity mid-chain
+        [(10, 0), (11, 0), (10, 1), (11, 1)],  # back to infinity
+        [(12, 0), (13, 0), (12, 0), (12, 0)],  # a point met again later
+        [(rng.randrange(24), rng.randrange(2)) for _ in range(60)],
+    ]
+    for chain in chains:
+        arr = np.ascontiguousarray(np.stack([np.asarray(pts[i], dtype=np.uint32) for i, _ in chain]), dtype=np.uint32)
+        neg = (ctypes.c_int * len(chain))(*[ng for _, ng in chain])
+        want = None
+        for i, ng in chain:
+            x, y = decode(pts[i])
+            want = _affine_add(want, (x, (P - y) % P if ng else y))
+        out = np.zeros(32, dtype=np.uint32)
+        empty, bounds = ctypes.c_int(), ctypes.c_int()
+        h29.h29_madd_chain(p_(arr), neg, len(chain), p_(out), ctypes.byref(empty), ctypes.byref(bounds))
+        assert bounds.value == 1, chain
         if want is None:
             assert empty.value == 1
             continue
