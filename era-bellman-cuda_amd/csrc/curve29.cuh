@@ -39,6 +39,25 @@ BC_HD xyzz29 xyzz29_from_xyzz(const g1_xyzz &p) {
 // back to the library's format (every coordinate below 2p)
 BC_HD g1_xyzz xyzz29_to_xyzz(const xyzz29 &p) { return {to_ext<Fq>(p.x), to_ext<Fq>(p.y), to_ext<Fq>(p.zz), to_ext<Fq>(p.zzz)}; }
 
+// acc = 2 acc (dbl-2008-s-1 with a = 0, 6M + 3S; the lazy Y3 as one product
+// pair), acc not infinity.  The additions below take this path when their
+// operands are equal.
+BC_HD void xyzz29_dbl(xyzz29 &acc) {
+  using P = P29<Fq>;
+  f29 V, XX, W, S, ZZ3, ZZZ3;
+  const f29 U = norm29(add29(acc.y, acc.y)); // 2 Y1 < 2^258.3
+  sqr29x2<Fq>(U, acc.x, V, XX);
+  mul29x2<Fq>(U, V, acc.x, V, W, S);
+  const f29 M = norm29(add29(XX, add29(XX, XX))); // 3 X1^2 < 2^258.9
+  const f29 MM = sqr29<Fq>(M);
+  const f29 X3 = reduce_value29<Fq>(norm29(sub29(MM, add29(S, S), P::c48e2)));
+  const f29 SX = norm29(sub29(S, X3, P::c2));
+  const f29 NY = norm29(neg29(acc.y, P::c16));
+  const f29 Y3 = mul2_29<Fq>(M, SX, W, NY); // M (S - X3) - W Y1
+  mul29x2<Fq>(V, acc.zz, W, acc.zzz, ZZ3, ZZZ3);
+  acc = {X3, Y3, ZZ3, ZZZ3};
+}
+
 // acc += q (add-2008-s, 12M + 2S), acc not infinity; q in the library's
 // format with every coordinate below 2p, not infinity.  `empty` is set when
 // the sum is infinity.
@@ -52,8 +71,7 @@ BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
   const f29 R = norm29(sub29(S2, S1, P::c16));  // S2 + 16m - S1
   if (is_zero29<Fq>(Pd)) {
     if (is_zero29<Fq>(R)) {
-      // acc equals q: 2 q by the formulas of the 2^256 radix
-      acc = xyzz29_from_xyzz(xyzz_dbl(qe));
+      xyzz29_dbl(acc); // acc equals q
     } else {
       empty = true; // acc = -q
     }
@@ -93,11 +111,7 @@ BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool 
   const f29 R = norm29(sub29(S2, acc.y, P::c16)); // S2 + 16m - Y1
   if (is_zero29<Fq>(Pd)) {
     if (is_zero29<Fq>(R)) {
-      // acc equals the point: twice the point by the formulas of the 2^256 radix
-      g1_affine qs = q;
-      if (negate)
-        qs.y = neg2<Fq>(q.y);
-      acc = xyzz29_from_xyzz(xyzz_dbl_affine(qs));
+      xyzz29_dbl(acc); // acc equals the point
     } else {
       empty = true; // acc = -point
     }
@@ -125,6 +139,20 @@ BC_HD void xyzz29_accumulate(xyzz29 &acc, bool &empty, const g1_xyzz &q) {
     return;
   }
   xyzz29_add(acc, empty, q);
+}
+
+// stores acc (infinity when `empty`) in the library's format, one coordinate
+// at a time (converting all four at once raised the accumulation kernel's
+// register use past three waves per SIMD)
+__device__ __forceinline__ void store_xyzz29(g1_xyzz *p, const xyzz29 &acc, bool empty) {
+  if (empty) {
+    store_xyzz(p, xyzz_infinity());
+    return;
+  }
+  store_fe(&p->x, to_ext<Fq>(acc.x));
+  store_fe(&p->y, to_ext<Fq>(acc.y));
+  store_fe(&p->zz, to_ext<Fq>(acc.zz));
+  store_fe(&p->zzz, to_ext<Fq>(acc.zzz));
 }
 
 } // namespace bc

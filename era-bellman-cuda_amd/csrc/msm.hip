@@ -293,47 +293,54 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   // the running sum in the 29-bit-limb form (infinity while `empty`)
   xyzz29 acc{};
   bool empty = true;
-  // software pipeline: entries three ahead, their points two ahead (a zero
-  // digit's point is not loaded); indexes past the chunk repeat its last entry
+  // software pipeline: entries three ahead, their points one ahead (a zero
+  // digit's point is not loaded; a second point in flight cost the registers
+  // of the third wave per SIMD); indexes past the chunk repeat its last entry
   const uint32_t last = end - 1;
   auto point_of = [&](uint64_t ent, const g1_affine &keep) {
     return (uint32_t)ent != top ? load_affine(bases + ((uint32_t)(ent >> 32) & ~SIGN)) : keep;
   };
   uint64_t ent0 = sorted[begin], ent1 = sorted[min(begin + 1, last)], ent2 = sorted[min(begin + 2, last)];
-  // the keys around the chunk, for the runs of the buckets it shares with its
-  // neighbours (not read at a job's ends, where a segment ends anyway)
-  uint32_t key_prev = begin != jd.entry_begin ? (uint32_t)sorted[begin - 1] : NONE;
-  const uint32_t key_after = end != job_end ? (uint32_t)sorted[end] : NONE;
-  g1_affine pt0 = point_of(ent0, g1_affine{}), pt1 = point_of(ent1, pt0);
+  // bucket b occupies [run_start[b], run_end[b]) of the sorted entries; a run
+  // is recorded by the chunk that holds its first (last) entry.  The chunk's
+  // first bucket continues a run when the entry before it, in the same
+  // segment, has its key; its last bucket continues into the next chunk alike.
+  const bool starts_segment = ((begin - jd.entry_begin) & seg_mask) == 0;
+  const bool continued = !starts_segment && (uint32_t)sorted[begin - 1] == (uint32_t)ent0;
+  const bool next_starts_segment = end == job_end || ((end - jd.entry_begin) & seg_mask) == 0;
+  const uint32_t key_after = next_starts_segment ? NONE : (uint32_t)sorted[end];
+  uint32_t last_e = 0, cur_key = NONE;
+  g1_affine pt0 = point_of(ent0, g1_affine{});
   for (uint32_t e = begin; e < end; e++) {
     const uint64_t ent3 = sorted[min(e + 3, last)];
-    const g1_affine pt2 = point_of(ent2, pt1);
+    const g1_affine pt1 = point_of(ent1, pt0);
     const uint32_t key = (uint32_t)ent0;
     if (key != top) { // zero digits add nothing
       const uint32_t b = jd.bucket_begin + (jd.stride ? 0u : (e - jd.entry_begin) >> jd.log_n) * top + key;
-      // bucket b occupies [run_start[b], run_end[b]) of the sorted entries
-      const uint32_t key_next = e != last ? (uint32_t)ent1 : key_after;
-      if (((e - jd.entry_begin) & seg_mask) == 0 || key_prev != key)
-        run_start[b] = e;
-      if (((e + 1 - jd.entry_begin) & seg_mask) == 0 || key_next != key)
-        run_end[b] = e + 1;
       if (b != cur) {
-        if (cur != NONE)
-          store_xyzz(slots + cur, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
+        if (cur != NONE) {
+          run_end[cur] = last_e + 1;
+          store_xyzz29(slots + cur, acc, empty);
+        }
+        if (e != begin || !continued)
+          run_start[b] = e;
         empty = true;
         cur = b;
+        cur_key = key;
       }
       xyzz29_add_affine(acc, empty, pt0, ((uint32_t)(ent0 >> 32) & SIGN) != 0);
+      last_e = e;
     }
-    key_prev = key;
     ent0 = ent1;
     ent1 = ent2;
     ent2 = ent3;
     pt0 = pt1;
-    pt1 = pt2;
   }
-  if (cur != NONE)
-    store_xyzz(slots + cur, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
+  if (cur != NONE) {
+    if (last_e + 1 != end || key_after != cur_key)
+      run_end[cur] = last_e + 1;
+    store_xyzz29(slots + cur, acc, empty);
+  }
 }
 
 // Heavy buckets are reduced in chunks of HEAVY_CHUNK partial sums, one block
