@@ -6,10 +6,11 @@
 // On the device the products are the generated assembly of mont29_asm.h, and
 // independent products go in pairs (mul29x2, sqr29x2) whose column chains
 // interleave.  Measured on MI355X: a register-only chain of bucket += point
-// runs at 17.8 G additions/s against 14.5 G for the 32-bit-limb madd; in the
-// 2^24 accumulation kernel (random 64-byte point loads, two waves per SIMD at
-// 217 VGPRs, and a clock that drops from about 2.0 to 1.9 GHz under the denser
-// multiplies) the gain is 2-4 % at c = 20 and 7 % at c = 17; the bucket-
+// runs at 17.8 G additions/s against 14.5 G for the 32-bit-limb madd (15.0 G
+// with every point loaded at random from a 1 GiB table); the 2^24
+// accumulation kernel is 5 % faster at c = 20 and 8 % at c = 17 than on
+// 32-bit limbs, at three waves per SIMD (166 VGPRs: hence the doubling on
+// 29-bit limbs and the one-coordinate-at-a-time stores); the bucket-
 // reduction split kernel runs 3-4 % faster than on the compiler's 29-bit
 // products.  The small-bucket gather stays on 32-bit limbs (0.84 -> 0.95 ms
 // on 29-bit limbs: most of its buckets add one or two terms, so the
