@@ -447,12 +447,17 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *parti
 // holds the bucket of the window's top digit (2^(c-1), or 2^top_signed in the last window).
 // With `add`, the buckets of this launch are added to what A and tops already
 // hold (the chunks of one MSM accumulate into the same windows).
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *partials,
-                                                                  const uint32_t *piece_offset, const uint32_t *piece_count, g1_xyzz *A,
-                                                                  g1_xyzz *tops, bool add) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nb)
-    return;
+// One bucket of the gather: where its sum goes and which partial sums make
+// it.  With `first`, the window's infinity stores are made here.
+struct gather_item {
+  g1_xyzz *dst;
+  uint32_t off, np;
+  bool live; // false: nothing to store for this bucket
+};
+
+__device__ __forceinline__ gather_item gather_prepare(const job_table &jobs, uint32_t t, const uint32_t *__restrict__ piece_offset,
+                                                      const uint32_t *__restrict__ piece_count, g1_xyzz *__restrict__ A,
+                                                      g1_xyzz *__restrict__ tops, bool add, bool first) {
   const job_desc &jd = jobs.job[find_job(jobs, t, by_bucket())];
   const uint32_t B = 1u << (jd.c - 1);
   const uint32_t local = t - jd.bucket_begin, w = local >> (jd.c - 1), k = local & (B - 1);
@@ -464,29 +469,81 @@ __global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_tabl
   // to the window's top when that is its top digit, and the top digit's own
   // slot in the last window hands it there instead
   const bool to_top = k == 0 ? top_digit == B : k == top_digit;
-  if ((k == 0 || k == top_digit) && !add)
+  if ((k == 0 || k == top_digit) && !add && first)
     store_xyzz(A + t, xyzz_infinity());
+  gather_item g{nullptr, 0, 0, false};
   if (k == 0 && !to_top)
-    return;
-  // bucket of digit d is batch bucket bucket_begin + w B + d - 1; buckets of
-  // 2..SMALL_PIECES sums are summed here, a heavy bucket's sum is already in
-  // its first slot
+    return g;
+  // bucket of digit d is batch bucket bucket_begin + w B + d - 1
   const uint32_t d = k == 0 ? B : k;
   const uint32_t b = jd.bucket_begin + w * B + d - 1;
-  const uint32_t np = piece_count[b], off = piece_offset[b];
-  g1_xyzz *dst = to_top ? tops + jd.top_begin + w : A + t;
-  if (add && np == 0)
-    return; // nothing of this launch to add
+  g.np = piece_count[b];
+  g.off = piece_offset[b];
+  g.dst = to_top ? tops + jd.top_begin + w : A + t;
+  g.live = !(add && g.np == 0); // nothing of this launch to add
+  return g;
+}
+
+// buckets of 2..SMALL_PIECES sums are summed here, a heavy bucket's sum is
+// already in its first slot
+__device__ __forceinline__ bool gather_sums(const gather_item &g) { return g.live && g.np >= 2 && g.np <= SMALL_PIECES; }
+
+__device__ __forceinline__ void gather_finish(const gather_item &g, const g1_xyzz *__restrict__ partials, bool add) {
   g1_xyzz acc = xyzz_infinity();
-  if (np != 0) {
-    acc = load_xyzz(partials + off);
-    if (np <= SMALL_PIECES)
-      for (uint32_t q = 1; q < np; q++)
-        acc = xyzz_add(acc, load_xyzz(partials + off + q));
+  if (g.np != 0) {
+    acc = load_xyzz(partials + g.off);
+    if (g.np <= SMALL_PIECES)
+      for (uint32_t q = 1; q < g.np; q++)
+        acc = xyzz_add(acc, load_xyzz(partials + g.off + q));
   }
   if (add)
-    acc = xyzz_add(load_xyzz(dst), acc);
-  store_xyzz(dst, acc);
+    acc = xyzz_add(load_xyzz(g.dst), acc);
+  store_xyzz(g.dst, acc);
+}
+
+// A thread per bucket.  With LIST, for launches whose buckets are much smaller
+// than the accumulation chunks (2^24 at c = 20: one bucket in six spans two
+// chunks), a thread copies a bucket of at most one partial sum at once and, in
+// a wave where few buckets have more, lists those in LDS; the block's threads
+// then sum the listed buckets, one per thread, so the additions run in waves
+// whose lanes all add (0.84 -> 0.63 ms): with a bucket per lane throughout,
+// almost every wave made 63 lanes wait for one lane's addition.  Where most
+// buckets span several chunks (2^24 at c = 17) the list saves nothing and its
+// block barriers keep finished waves resident (0.26 -> 0.29 ms), so such
+// launches sum every bucket in place.
+constexpr unsigned GATHER_LIST_MAX_LANES = 16; // a wave lists when at most this many of its lanes add
+
+template <bool LIST>
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *__restrict__ partials,
+                                                                  const uint32_t *__restrict__ piece_offset,
+                                                                  const uint32_t *__restrict__ piece_count, g1_xyzz *__restrict__ A,
+                                                                  g1_xyzz *__restrict__ tops, bool add) {
+  const uint32_t t = blockIdx.x * MSM_BLOCK + threadIdx.x;
+  if (!LIST) {
+    if (t < nb) {
+      const gather_item g = gather_prepare(jobs, t, piece_offset, piece_count, A, tops, add, true);
+      if (g.live)
+        gather_finish(g, partials, add);
+    }
+    return;
+  }
+  __shared__ uint32_t list[MSM_BLOCK];
+  __shared__ uint32_t listed;
+  if (threadIdx.x == 0)
+    listed = 0;
+  __syncthreads();
+  gather_item g{nullptr, 0, 0, false};
+  if (t < nb)
+    g = gather_prepare(jobs, t, piece_offset, piece_count, A, tops, add, true);
+  const bool sums = gather_sums(g);
+  const bool in_place = __popcll(__ballot(sums)) > GATHER_LIST_MAX_LANES;
+  if (g.live && (!sums || in_place))
+    gather_finish(g, partials, add);
+  else if (sums)
+    list[atomicAdd(&listed, 1u)] = t;
+  __syncthreads();
+  if (threadIdx.x < listed)
+    gather_finish(gather_prepare(jobs, list[threadIdx.x], piece_offset, piece_count, A, tops, add, false), partials, add);
 }
 
 // 5. one split level for every job whose windows are still wider than one bit.
@@ -518,7 +575,13 @@ template <class Table> __device__ __forceinline__ unsigned find_seg(const Table 
   return lo;
 }
 
-__global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs, uint32_t total) {
+// PREFETCH: the next term is loaded before the current one is added (64 more
+// VGPRs: two waves per SIMD instead of three).  It pays when the launch is too
+// small to fill the GPU (2^24 at c = 17: 123 K threads, 0.27 -> 0.20 ms) and
+// costs when it is large (2^24 at c = 20: 1.7 M threads, 1.44 -> 1.50 ms).
+constexpr uint32_t SPLIT_PREFETCH_BELOW = 1u << 19;
+
+template <bool PREFETCH> __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs, uint32_t total) {
   const uint32_t id_all = blockIdx.x * blockDim.x + threadIdx.x;
   if (id_all >= total)
     return;
@@ -543,8 +606,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
   }
   const g1_xyzz *src = sg.A + ((size_t)x << s);
   // terms q = first, first + step, ... < limit at src[(q << t) | j] (low half)
-  // or src[(j << t) | q] (high half); the next term is loaded before the
-  // current one is added
+  // or src[(j << t) | q] (high half)
   const bool lo = half == 0;
   const uint32_t first = part, step = lo ? sg.D_lo : sg.D, limit = lo ? (1u << hs) : (1u << t);
   const bool active = lo ? part < sg.D_lo : j < (1u << hs);
@@ -554,18 +616,23 @@ __global__ __launch_bounds__(ACC_BLOCK) void split_kernel(const split_table segs
   bool empty = true;
   if (active && first < limit) {
     xyzz29_accumulate(acc, empty, load_xyzz(term(first)));
-    uint32_t q = first + step;
-    if (q < limit) {
-      g1_xyzz cur = load_xyzz(term(q));
-      for (q += step; q < limit; q += step) {
-        const g1_xyzz next = load_xyzz(term(q));
+    if (PREFETCH) {
+      uint32_t q = first + step;
+      if (q < limit) {
+        g1_xyzz cur = load_xyzz(term(q));
+        for (q += step; q < limit; q += step) {
+          const g1_xyzz next = load_xyzz(term(q));
+          xyzz29_accumulate(acc, empty, cur);
+          cur = next;
+        }
         xyzz29_accumulate(acc, empty, cur);
-        cur = next;
       }
-      xyzz29_accumulate(acc, empty, cur);
+    } else {
+      for (uint32_t q = first + step; q < limit; q += step)
+        xyzz29_accumulate(acc, empty, load_xyzz(term(q)));
     }
   }
-  store_xyzz(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, empty ? xyzz_infinity() : xyzz29_to_xyzz(acc));
+  store_xyzz29(sg.out + (size_t)part * per_part + ((2 * x + half) << t) + j, acc, empty);
 }
 
 // out[i] += out[i + half] for i < half, per segment
@@ -828,7 +895,20 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
   }
   BC_TRY(ar.put(heavy));
 
-  bucket_gather_kernel<<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops, add);
+  // the listing gather when most buckets lie in jobs whose buckets average at
+  // most a quarter of a chunk
+  uint64_t small_buckets = 0;
+  for (unsigned j = 0; j < count; j++) {
+    const job_desc &d = jt.job[j];
+    const uint64_t job_buckets = (uint64_t)d.W << (d.c - 1);
+    const uint64_t entries_per_bucket = ((uint64_t)(d.stride ? d.Wd : 1u) << d.log_n) >> (d.c - 1);
+    if (4 * entries_per_bucket <= d.piece)
+      small_buckets += job_buckets;
+  }
+  if (2 * small_buckets >= nb)
+    bucket_gather_kernel<true><<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops, add);
+  else
+    bucket_gather_kernel<false><<<grid_for(nb, MSM_BLOCK), MSM_BLOCK, 0, s>>>(jt, nb, partials, piece_offset, piece_count, A, tops, add);
   BC_LAUNCH_CHECK();
   for (void *p : {(void *)partials, (void *)piece_offset, (void *)piece_count, (void *)meta, (void *)offsets, (void *)counts,
                   (void *)sorted})
@@ -878,7 +958,10 @@ hipError_t reduce_phase(const batch_layout &bl, const msm_job *jobs, unsigned co
     level_buffers.push_back(next);
     for (unsigned k = 0; k < st.count; k++)
       st.seg[k].out = next + st.seg[k].thread_begin;
-    split_kernel<<<grid_for(threads, ACC_BLOCK), ACC_BLOCK, 0, s>>>(st, (uint32_t)threads);
+    if (threads < SPLIT_PREFETCH_BELOW)
+      split_kernel<true><<<grid_for(threads, ACC_BLOCK), ACC_BLOCK, 0, s>>>(st, (uint32_t)threads);
+    else
+      split_kernel<false><<<grid_for(threads, ACC_BLOCK), ACC_BLOCK, 0, s>>>(st, (uint32_t)threads);
     BC_LAUNCH_CHECK();
     // fold round r halves every segment that still holds more than one part
     for (unsigned r = 0;; r++) {
