@@ -157,6 +157,9 @@ def test_emulated_products_are_montgomery_products():
         full = [M29] * 8 + [(1 << 28) - 1]
         check("mul29_device", fns["mul29_device"], m, {"a": wide, "b": full})
         check("mul29x2_device", fns["mul29x2_device"], m, {"a": wide, "b": full, "c": full, "e": wide})
+        # operands below 2^261, every limb at 2^29 - 1
+        top29 = [M29] * 9
+        check("mul29x2_device", fns["mul29x2_device"], m, {"a": top29, "b": full, "c": top29, "e": top29})
         check("mul2_29_device", fns["mul2_29_device"], m, {"a": full, "b": full, "c": full, "e": full})
         check("sqr29_device", fns["sqr29_device"], m, {"a": full})
         check("sqr29x2_device", fns["sqr29x2_device"], m, {"a": full, "c": full})
