@@ -653,7 +653,12 @@ __global__ __launch_bounds__(ACC_BLOCK) void fold_kernel(const fold_table segs, 
     return;
   const fold_seg &sg = segs.seg[find_seg(segs, id_all)];
   const uint32_t i = id_all - sg.thread_begin;
-  store_xyzz(sg.out + i, xyzz_add(load_xyzz(sg.out + i), load_xyzz(sg.out + i + sg.half)));
+  // the sum on 29-bit limbs, as in the split kernel
+  xyzz29 acc;
+  bool empty = true;
+  xyzz29_accumulate(acc, empty, load_xyzz(sg.out + i));
+  xyzz29_accumulate(acc, empty, load_xyzz(sg.out + i + sg.half));
+  store_xyzz29(sg.out + i, acc, empty);
 }
 
 // 6. bit i = w*c + j of a job is its window's top bucket (j == c-1 or i == 253),
