@@ -489,6 +489,17 @@ __device__ __forceinline__ gather_item gather_prepare(const job_table &jobs, uin
 __device__ __forceinline__ bool gather_sums(const gather_item &g) { return g.live && g.np >= 2 && g.np <= SMALL_PIECES; }
 
 __device__ __forceinline__ void gather_finish(const gather_item &g, const g1_xyzz *__restrict__ partials, bool add) {
+  if (g.np >= 2 && g.np <= SMALL_PIECES) {
+    // several sums: added on 29-bit limbs
+    xyzz29 acc;
+    bool empty = true;
+    for (uint32_t q = 0; q < g.np; q++)
+      xyzz29_accumulate(acc, empty, load_xyzz(partials + g.off + q));
+    if (add)
+      xyzz29_accumulate(acc, empty, load_xyzz(g.dst));
+    store_xyzz29(g.dst, acc, empty);
+    return;
+  }
   g1_xyzz acc = xyzz_infinity();
   if (g.np != 0) {
     acc = load_xyzz(partials + g.off);
