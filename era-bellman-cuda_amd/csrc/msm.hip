@@ -168,7 +168,7 @@ hipError_t fill_u32(uint32_t *p, size_t n, uint32_t v, hipStream_t s) {
 __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs, uint64_t *__restrict__ entries, uint32_t *__restrict__ first_table,
                                                            unsigned bits0, uint32_t block_points) {
   using F = FrParams;
-  // [W][2^min(c, bits0)] 16-bit counts (a tile has SEG_TILE entries), two per
+  // [Wd][2^min(c, bits0)] 16-bit counts (a tile has SEG_TILE entries), two per
   // word: digit d of window w is half d & 1 of word (w nd + d) / 2
   extern __shared__ uint32_t hist[];
   // blocks are dealt round-robin to the 8 XCDs; block ids are remapped so each
@@ -177,7 +177,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
   const uint32_t per_xcd = gridDim.x >> 3;
   const uint32_t blk = blockIdx.x < (per_xcd << 3) ? (blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
   const job_desc &jd = jobs.job[find_job(jobs, blk, by_digit_block())];
-  const uint32_t k = blk - jd.digit_block, n = 1u << jd.log_n, c = jd.c, W = jd.W, Wd = jd.Wd;
+  const uint32_t k = blk - jd.digit_block, n = 1u << jd.log_n, c = jd.c, Wd = jd.Wd;
   const uint32_t top = 1u << (c - 1);
   const unsigned dbits = c < bits0 ? c : bits0; // keys are below 2^c
   const uint32_t nd = 1u << dbits;
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
     atomicAdd(&hist[slot >> 1], v << ((slot & 1) << 4));
   };
   if (first_table) {
-    for (uint32_t i = threadIdx.x; i < (W * nd + 1) / 2; i += MSM_BLOCK)
+    for (uint32_t i = threadIdx.x; i < (Wd * nd + 1) / 2; i += MSM_BLOCK)
       hist[i] = 0;
     __syncthreads();
   }
@@ -228,11 +228,15 @@ __global__ __launch_bounds__(MSM_BLOCK) void digits_kernel(const job_table jobs,
   }
   if (first_table) {
     __syncthreads();
-    // window w, digit d of tile k; digits at or above 2^c count nothing
+    // window w, digit d of tile k (digits at or above 2^c count nothing): the
+    // table is [window][digit][tile] over the windows' segments, or, for a
+    // fixed-base job's single segment (windows of whole tiles, n >= SEG_TILE),
+    // [digit][w tiles + k]
     const uint32_t tiles = (n + SEG_TILE - 1) / SEG_TILE;
-    for (uint32_t j = threadIdx.x; j < (W << bits0); j += MSM_BLOCK) {
+    for (uint32_t j = threadIdx.x; j < (Wd << bits0); j += MSM_BLOCK) {
       const uint32_t w = j >> bits0, d = j & ((1u << bits0) - 1), slot = w * nd + d;
-      first_table[jd.table_begin + (uint64_t)j * tiles + k] = d < nd ? (hist[slot >> 1] >> ((slot & 1) << 4)) & 0xffffu : 0u;
+      const uint64_t at = jd.stride ? (uint64_t)d * Wd * tiles + w * tiles + k : (uint64_t)j * tiles + k;
+      first_table[jd.table_begin + at] = d < nd ? (hist[slot >> 1] >> ((slot & 1) << 4)) & 0xffffu : 0u;
     }
   }
 }
@@ -843,24 +847,25 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
     BC_TRY(ar.get(&digits, entries));
     seg_group groups[MSM_MAX_JOBS];
     unsigned key_bits = 0; // local keys are <= 2^(c-1)
-    bool fixed_base = false;
+    bool unaligned = false; // a fixed-base job whose windows are not whole sort tiles
     for (unsigned j = 0; j < count; j++) {
       // a sort segment per window, or one over all of a fixed-base job's entries
       const job_desc &d = jt.job[j];
       groups[j] = d.stride ? seg_group{d.entry_begin, (uint32_t)d.Wd << d.log_n, 1u} : seg_group{d.entry_begin, 1u << d.log_n, d.W};
       key_bits = std::max<unsigned>(key_bits, d.c);
-      fixed_base |= d.stride != 0;
+      unaligned |= d.stride != 0 && (1u << d.log_n) < SEG_TILE;
     }
     // the digits kernel counts the sort's first digit when its per-window
     // counters fit DIGITS_MAX_LDS; otherwise the sort counts them itself
     seg_first_pass fp;
     BC_TRY(segmented_sort_first_pass(groups, count, key_bits, &fp));
-    size_t lds = 0; // 16-bit counters, two per word
+    size_t lds = 0; // 16-bit counters per digit window, two per word
     for (unsigned j = 0; j < count; j++)
-      lds = std::max(lds, sizeof(uint32_t) * ((((size_t)jt.job[j].W << std::min<unsigned>(jt.job[j].c, fp.bits)) + 1) / 2));
+      lds = std::max(lds, sizeof(uint32_t) * ((((size_t)jt.job[j].Wd << std::min<unsigned>(jt.job[j].c, fp.bits)) + 1) / 2));
     uint32_t *first_table = nullptr;
-    // (the digits kernel counts per window tile: not for a fixed-base job's single segment)
-    if (fp.bits > 0 && lds <= DIGITS_MAX_LDS && fp.table_entries < (1ull << 32) && !fixed_base) {
+    // (the digits kernel counts per window tile: a fixed-base job's single
+    // segment qualifies when its windows are whole tiles)
+    if (fp.bits > 0 && lds <= DIGITS_MAX_LDS && fp.table_entries < (1ull << 32) && !unaligned) {
       BC_TRY(ar.get(&first_table, fp.table_entries));
       for (unsigned j = 0; j < count; j++)
         jt.job[j].table_begin = (uint32_t)fp.table_begin[j];
