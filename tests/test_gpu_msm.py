@@ -237,6 +237,43 @@ def _host_buffer(B, arr, pinned):
     return p.value, None, lambda: B.call("bc_free_host", p)
 
 
+@pytest.mark.timeout(300)
+def test_msm_host_scalars_chunks_add_listed_buckets(bellman, oracle):
+    """Host scalars at 2^23: four chunks of 2^21 points accumulate into the same
+    windows at the full size's width (c = 19).  Each chunk's buckets hold about
+    8 entries against accumulation chunks of 32, so the gather lists its
+    buckets of several partial sums in LDS and adds them, with the single-sum
+    ones, to what the earlier chunks left in the windows.  The per-bit results
+    equal the device-input call's as points and the fold is the closed form."""
+    import ctypes
+
+    B, dev = bellman
+    L = 23
+    n = 1 << L
+    rng = np.random.default_rng(0xBC000023)
+    s = rng.integers(0, 1 << 32, size=(n, 8), dtype=np.uint32)
+    s[:, 7] %= 0x30644E72  # below r's top word: every value is below r
+    k = oracle.msm_scalars(0xBC000024, n)
+    bufs = []
+    try:
+        d_s = dev.upload(s)
+        d_b = dev.malloc(64 * n)
+        bufs += [d_s, d_b]
+        B.call("bc_amd_generator_multiples", ctypes.c_void_p(d_s), ctypes.c_void_p(d_b), ctypes.c_uint(n), dev.stream)
+        r2 = oracle.to_limbs([(1 << 512) % oracle.R])[0]  # Montgomery form: v R = mont_mul(R^2, v)
+        B.call("ff_ax", B.ptr(r2), ctypes.c_void_p(d_s), ctypes.c_void_p(d_s), ctypes.c_uint(n), dev.stream)
+        dev.sync()
+        s_mont = dev.download(d_s, (n, 8))
+        want = msm(dev, d_b, k, L)
+        got = msm(dev, d_b, k, L, host_scalars=True)
+        assert np.array_equal(oracle.to_affine(got), oracle.to_affine(want))
+        t = oracle.from_mont(oracle.fr_dot(k, s_mont))[0]
+        assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], oracle.mul_generator(t))
+    finally:
+        for p in bufs:
+            dev.free(p)
+
+
 @pytest.mark.parametrize("bases_where,scalars_where", [("pageable", "device"), ("pinned", "device"), ("device", "pinned"), ("pinned", "pageable"),
                                                        ("pageable", "pageable")])
 def test_msm_host_inputs_chunked(bellman, oracle, bases_where, scalars_where):
