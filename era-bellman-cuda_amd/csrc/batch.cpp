@@ -188,14 +188,14 @@ hipError_t run_in_block(hipMemPool_t pool, hipStream_t s, size_t bytes, const st
 // tile-histogram tables (the first pass's, counted by the digits kernel, and
 // the sort's own for later passes: each at most 2 B per entry plus 8 KiB per
 // window), a
-// 128 B partial-sum slot per bucket and per chunk of `piece` entries (132 B
-// counted), per bucket the run/slot tables, the heavy-bucket list and the
+// 144 B partial-sum slot (29-bit limbs) per bucket and per chunk of `piece`
+// entries (148 B counted), per bucket the run/slot tables, the heavy-bucket list and the
 // level-0 window array (152 B), and 128 B per bucket-split thread, with 256 B
 // of rounding per split level.
 size_t msm_footprint(const msm_job &j) {
   const uint64_t Wd = (254 - 1) / j.c + 1, W = j.stride ? 1 : Wd; // digit and bucket windows
   const uint64_t E = Wd << j.log_n, nb = W << (j.c - 1);
-  return (size_t)(28 * E + 16384 * W + 132 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c, (unsigned)W) + 256 * 8 + (1u << 20));
+  return (size_t)(28 * E + 16384 * W + 148 * (E / j.piece + nb + 1) + 152 * nb + 128 * msm_split_threads(j.c, (unsigned)W) + 256 * 8 + (1u << 20));
 }
 
 // Coset transforms hold one twist table (2^12 fine + 2^(L-12) coarse entries).

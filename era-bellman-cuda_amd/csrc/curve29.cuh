@@ -59,15 +59,14 @@ BC_HD void xyzz29_dbl(xyzz29 &acc) {
   acc = {X3, Y3, ZZ3, ZZZ3};
 }
 
-// acc += q (add-2008-s, 12M + 2S), acc not infinity; q in the library's
-// format with every coordinate below 2p, not infinity.  `empty` is set when
-// the sum is infinity.
-BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
+// acc += q (add-2008-s, 12M + 2S), acc not infinity; q not infinity, its
+// coordinates normalized and below 2^259.6 (a partial sum on 29-bit limbs, or
+// the library's format regrouped).  `empty` is set when the sum is infinity.
+BC_HD void xyzz29_add29(xyzz29 &acc, bool &empty, const xyzz29 &q) {
   using P = P29<Fq>;
-  const f29 X2 = ext_to29(qe.x), Y2 = ext_to29(qe.y), ZZ2 = ext_to29(qe.zz), ZZZ2 = ext_to29(qe.zzz); // < 2^259.6
   f29 U1, U2, S1, S2;
-  mul29x2<Fq>(acc.x, ZZ2, X2, acc.zz, U1, U2);
-  mul29x2<Fq>(acc.y, ZZZ2, Y2, acc.zzz, S1, S2);
+  mul29x2<Fq>(acc.x, q.zz, q.x, acc.zz, U1, U2);
+  mul29x2<Fq>(acc.y, q.zzz, q.y, acc.zzz, S1, S2);
   const f29 Pd = norm29(sub29(U2, U1, P::c16)); // U2 + 16m - U1
   const f29 R = norm29(sub29(S2, S1, P::c16));  // S2 + 16m - S1
   if (is_zero29<Fq>(Pd)) {
@@ -86,9 +85,15 @@ BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
   const f29 QX = norm29(sub29(Q, X3, P::c2));
   const f29 NP = norm29(neg29(PPP, P::c16));
   const f29 Y3 = mul2_29<Fq>(R, QX, S1, NP); // R (Q - X3) - S1 PPP
-  mul29x2<Fq>(ZZ2, PP, ZZZ2, PPP, T, TT);
+  mul29x2<Fq>(q.zz, PP, q.zzz, PPP, T, TT);
   mul29x2<Fq>(acc.zz, T, acc.zzz, TT, ZZ3, ZZZ3);
   acc = {X3, Y3, ZZ3, ZZZ3};
+}
+
+// the same with q in the library's format, every coordinate below 2p (below
+// 2^259.6 once regrouped)
+BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
+  xyzz29_add29(acc, empty, {ext_to29(qe.x), ext_to29(qe.y), ext_to29(qe.zz), ext_to29(qe.zzz)});
 }
 
 // acc += (x, y) or (x, -y) (madd-2008-s, 8M + 2S with the lazy Y3 as one
@@ -130,6 +135,23 @@ BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool 
   acc = {X3, Y3, ZZ3, ZZZ3};
 }
 
+// Partial sums kept on 29-bit limbs in memory: 4 x 9 words (144 bytes, the
+// accumulators' bounds), infinity when ZZ = 0 (mod p).
+BC_HD xyzz29 xyzz29_infinity() { return {}; }
+BC_HD bool xyzz29_is_infinity(const xyzz29 &p) { return is_zero29<Fq>(p.zz); }
+
+// acc += q, q a partial sum on 29-bit limbs; either may be infinity
+BC_HD void xyzz29_accumulate29(xyzz29 &acc, bool &empty, const xyzz29 &q) {
+  if (xyzz29_is_infinity(q))
+    return;
+  if (empty) {
+    acc = q;
+    empty = false;
+    return;
+  }
+  xyzz29_add29(acc, empty, q);
+}
+
 // acc += q where either may be infinity (`empty`, or q's ZZ = 0 mod p)
 BC_HD void xyzz29_accumulate(xyzz29 &acc, bool &empty, const g1_xyzz &q) {
   if (xyzz_is_infinity(q))
@@ -140,6 +162,44 @@ BC_HD void xyzz29_accumulate(xyzz29 &acc, bool &empty, const g1_xyzz &q) {
     return;
   }
   xyzz29_add(acc, empty, q);
+}
+
+// a partial sum on 29-bit limbs: nine 16-byte accesses (p 16-byte aligned)
+__device__ __forceinline__ xyzz29 load_xyzz29(const xyzz29 *p) {
+  const uint4 *q = reinterpret_cast<const uint4 *>(p);
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    const uint4 v = q[i];
+    w[4 * i] = v.x;
+    w[4 * i + 1] = v.y;
+    w[4 * i + 2] = v.z;
+    w[4 * i + 3] = v.w;
+  }
+  xyzz29 r;
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    r.x.v[i] = w[i];
+    r.y.v[i] = w[9 + i];
+    r.zz.v[i] = w[18 + i];
+    r.zzz.v[i] = w[27 + i];
+  }
+  return r;
+}
+
+__device__ __forceinline__ void store_xyzz29_raw(xyzz29 *p, const xyzz29 &v) {
+  uint32_t w[36];
+#pragma unroll
+  for (int i = 0; i < 9; i++) {
+    w[i] = v.x.v[i];
+    w[9 + i] = v.y.v[i];
+    w[18 + i] = v.zz.v[i];
+    w[27 + i] = v.zzz.v[i];
+  }
+  uint4 *q = reinterpret_cast<uint4 *>(p);
+#pragma unroll
+  for (int i = 0; i < 9; i++)
+    q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
 }
 
 // stores acc (infinity when `empty`) in the library's format, one coordinate

@@ -277,7 +277,7 @@ __global__ void bucket_slots_kernel(const job_table jobs, const uint32_t *start,
 }
 
 __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table jobs, const uint64_t *__restrict__ sorted, uint32_t chunks,
-                                                               g1_xyzz *__restrict__ partials, uint32_t *__restrict__ run_start,
+                                                               xyzz29 *__restrict__ partials, uint32_t *__restrict__ run_start,
                                                                uint32_t *__restrict__ run_end) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= chunks)
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   // sort segments: every window, or all of a fixed-base job's entries;
   // (e - entry_begin) & seg_mask is 0 exactly at a segment's first entry
   const uint32_t seg_mask = jd.stride ? 0xffffffffu : (1u << jd.log_n) - 1;
-  g1_xyzz *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
+  xyzz29 *slots = partials + (jd.slot_begin + k - jd.bucket_begin); // + batch-global bucket
   constexpr uint32_t NONE = 0xffffffffu;
   uint32_t cur = NONE;
   // the running sum in the 29-bit-limb form (infinity while `empty`)
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
       if (b != cur) {
         if (cur != NONE) {
           run_end[cur] = last_e + 1;
-          store_xyzz29(slots + cur, acc, empty);
+          store_xyzz29_raw(slots + cur, empty ? xyzz29_infinity() : acc);
         }
         if (e != begin || !continued)
           run_start[b] = e;
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
   if (cur != NONE) {
     if (last_e + 1 != end || key_after != cur_key)
       run_end[cur] = last_e + 1;
-    store_xyzz29(slots + cur, acc, empty);
+    store_xyzz29_raw(slots + cur, empty ? xyzz29_infinity() : acc);
   }
 }
 
@@ -382,10 +382,10 @@ __global__ __launch_bounds__(256) void heavy_prefix_kernel(const uint32_t *heavy
 }
 
 // 4a. one block per chunk (grid-stride over all chunks of all heavy buckets)
-__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(xyzz29 *partials, const uint32_t *heavy, const uint32_t *heavy_count,
                                                                   const uint32_t *chunk_offset, const uint32_t *chunk_total,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count) {
-  __shared__ g1_xyzz sh[HEAVY_BLOCK];
+  __shared__ xyzz29 sh[HEAVY_BLOCK];
   const uint32_t total = *chunk_total, count = *heavy_count;
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
     uint32_t lo = 0, hi = count; // heavy bucket of chunk c
@@ -398,50 +398,59 @@ __global__ __launch_bounds__(HEAVY_BLOCK) void heavy_chunk_kernel(g1_xyzz *parti
     }
     const uint32_t b = heavy[lo];
     const uint32_t np = piece_count[b];
-    g1_xyzz *base = partials + piece_offset[b] + (size_t)(c - chunk_offset[lo]) * HEAVY_CHUNK;
+    xyzz29 *base = partials + piece_offset[b] + (size_t)(c - chunk_offset[lo]) * HEAVY_CHUNK;
     const uint32_t left = np - (c - chunk_offset[lo]) * HEAVY_CHUNK; // sums from the chunk start
     xyzz29 acc;
     bool empty = true;
     for (uint32_t k = 0; k < HEAVY_PER_THREAD; k++) {
       const uint32_t q = threadIdx.x * HEAVY_PER_THREAD + k;
       if (q < left)
-        xyzz29_accumulate(acc, empty, load_xyzz(base + q));
+        xyzz29_accumulate29(acc, empty, load_xyzz29(base + q));
     }
-    sh[threadIdx.x] = empty ? xyzz_infinity() : xyzz29_to_xyzz(acc);
+    sh[threadIdx.x] = empty ? xyzz29_infinity() : acc;
     __syncthreads();
     for (unsigned half = HEAVY_BLOCK / 2; half > 0; half >>= 1) {
-      if (threadIdx.x < half)
-        sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + half]);
+      if (threadIdx.x < half) {
+        xyzz29 a = sh[threadIdx.x];
+        bool e = xyzz29_is_infinity(a);
+        xyzz29_accumulate29(a, e, sh[threadIdx.x + half]);
+        sh[threadIdx.x] = e ? xyzz29_infinity() : a;
+      }
       __syncthreads();
     }
     if (threadIdx.x == 0)
-      store_xyzz(base, sh[0]);
+      store_xyzz29_raw(base, sh[0]);
     __syncthreads();
   }
 }
 
 // 4b. one block per heavy bucket (grid-stride): its chunk sums, strided over
 // the threads, then an LDS tree, into the first slot
-__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(g1_xyzz *partials, const uint32_t *heavy, const uint32_t *heavy_count,
+__global__ __launch_bounds__(HEAVY_BLOCK) void heavy_final_kernel(xyzz29 *partials, const uint32_t *heavy, const uint32_t *heavy_count,
                                                                   const uint32_t *piece_offset, const uint32_t *piece_count) {
-  __shared__ g1_xyzz sh[HEAVY_BLOCK];
+  __shared__ xyzz29 sh[HEAVY_BLOCK];
   const uint32_t count = *heavy_count;
   for (uint32_t h = blockIdx.x; h < count; h += gridDim.x) {
     const uint32_t b = heavy[h];
-    g1_xyzz *base = partials + piece_offset[b];
+    xyzz29 *base = partials + piece_offset[b];
     const uint32_t chunks = (piece_count[b] + HEAVY_CHUNK - 1) / HEAVY_CHUNK;
-    g1_xyzz acc = xyzz_infinity();
+    xyzz29 acc;
+    bool empty = true;
     for (uint32_t q = threadIdx.x; q < chunks; q += HEAVY_BLOCK)
-      acc = xyzz_add(acc, load_xyzz(base + (size_t)q * HEAVY_CHUNK));
-    sh[threadIdx.x] = acc;
+      xyzz29_accumulate29(acc, empty, load_xyzz29(base + (size_t)q * HEAVY_CHUNK));
+    sh[threadIdx.x] = empty ? xyzz29_infinity() : acc;
     __syncthreads();
     for (unsigned half = HEAVY_BLOCK / 2; half > 0; half >>= 1) {
-      if (threadIdx.x < half)
-        sh[threadIdx.x] = xyzz_add(sh[threadIdx.x], sh[threadIdx.x + half]);
+      if (threadIdx.x < half) {
+        xyzz29 a = sh[threadIdx.x];
+        bool e = xyzz29_is_infinity(a);
+        xyzz29_accumulate29(a, e, sh[threadIdx.x + half]);
+        sh[threadIdx.x] = e ? xyzz29_infinity() : a;
+      }
       __syncthreads();
     }
     if (threadIdx.x == 0)
-      store_xyzz(base, sh[0]);
+      store_xyzz29_raw(base, sh[0]);
     __syncthreads();
   }
 }
@@ -492,28 +501,20 @@ __device__ __forceinline__ gather_item gather_prepare(const job_table &jobs, uin
 // already in its first slot
 __device__ __forceinline__ bool gather_sums(const gather_item &g) { return g.live && g.np >= 2 && g.np <= SMALL_PIECES; }
 
-__device__ __forceinline__ void gather_finish(const gather_item &g, const g1_xyzz *__restrict__ partials, bool add) {
-  if (g.np >= 2 && g.np <= SMALL_PIECES) {
-    // several sums: added on 29-bit limbs
-    xyzz29 acc;
-    bool empty = true;
-    for (uint32_t q = 0; q < g.np; q++)
-      xyzz29_accumulate(acc, empty, load_xyzz(partials + g.off + q));
-    if (add)
-      xyzz29_accumulate(acc, empty, load_xyzz(g.dst));
-    store_xyzz29(g.dst, acc, empty);
-    return;
-  }
-  g1_xyzz acc = xyzz_infinity();
+__device__ __forceinline__ void gather_finish(const gather_item &g, const xyzz29 *__restrict__ partials, bool add) {
+  // the partial sums are on 29-bit limbs; the window arrays take the library's format
+  xyzz29 acc = xyzz29_infinity();
+  bool empty = true;
   if (g.np != 0) {
-    acc = load_xyzz(partials + g.off);
+    acc = load_xyzz29(partials + g.off);
+    empty = xyzz29_is_infinity(acc);
     if (g.np <= SMALL_PIECES)
       for (uint32_t q = 1; q < g.np; q++)
-        acc = xyzz_add(acc, load_xyzz(partials + g.off + q));
+        xyzz29_accumulate29(acc, empty, load_xyzz29(partials + g.off + q));
   }
   if (add)
-    acc = xyzz_add(load_xyzz(g.dst), acc);
-  store_xyzz(g.dst, acc);
+    xyzz29_accumulate(acc, empty, load_xyzz(g.dst));
+  store_xyzz29(g.dst, acc, empty);
 }
 
 // A thread per bucket.  With LIST, for launches whose buckets are much smaller
@@ -529,7 +530,7 @@ __device__ __forceinline__ void gather_finish(const gather_item &g, const g1_xyz
 constexpr unsigned GATHER_LIST_MAX_LANES = 16; // a wave lists when at most this many of its lanes add
 
 template <bool LIST>
-__global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const g1_xyzz *__restrict__ partials,
+__global__ __launch_bounds__(MSM_BLOCK) void bucket_gather_kernel(const job_table jobs, uint32_t nb, const xyzz29 *__restrict__ partials,
                                                                   const uint32_t *__restrict__ piece_offset,
                                                                   const uint32_t *__restrict__ piece_count, g1_xyzz *__restrict__ A,
                                                                   g1_xyzz *__restrict__ tops, bool add) {
@@ -888,7 +889,7 @@ hipError_t accumulate_phase(batch_layout &bl, unsigned count, arena &ar, g1_xyzz
   // bucket's run [offsets[b], counts[b]) (empty buckets keep [0, 0));
   // meta = {heavy count, heavy chunk total}
   uint32_t *piece_count = nullptr, *piece_offset = nullptr, *meta = nullptr, *heavy = nullptr;
-  g1_xyzz *partials = nullptr;
+  xyzz29 *partials = nullptr; // on 29-bit limbs, 144 B each
   BC_TRY(ar.get(&piece_count, nb));
   BC_TRY(ar.get(&piece_offset, nb));
   BC_TRY(ar.get(&meta, 2));
