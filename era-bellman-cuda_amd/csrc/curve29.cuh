@@ -96,18 +96,29 @@ BC_HD void xyzz29_add(xyzz29 &acc, bool &empty, const g1_xyzz &qe) {
   xyzz29_add29(acc, empty, {ext_to29(qe.x), ext_to29(qe.y), ext_to29(qe.zz), ext_to29(qe.zzz)});
 }
 
+// The point (x, y) as an XYZZ sum on 29-bit limbs (ZZ = ZZZ = 1).
+BC_HD xyzz29 xyzz29_point(const f29 &X, const f29 &Y) {
+  return {reduce_value29<Fq>(X), reduce_value29<Fq>(norm29(Y)), one29<Fq>(), one29<Fq>()};
+}
+
 // acc += (x, y) or (x, -y) (madd-2008-s, 8M + 2S with the lazy Y3 as one
 // product pair), for an affine point with canonical coordinates.  acc may be
 // infinity (`empty`); the point is not.  In the 2^261 radix the point's
 // coordinates are 32 x their 2^256-radix values, below 2^259.
-BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool negate) {
+// The general formula always overwrites acc, also when it does not apply
+// (equal x coordinates: acc was the point or its negation); that rare case
+// then reads the point again from `again` (the same point as q) and doubles
+// it, or makes acc infinity.  Keeping acc for it instead holds a second copy
+// of its four coordinates across the formula (the accumulation kernel: 158
+// against 135 VGPRs).
+BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool negate, const g1_affine *again) {
   using P = P29<Fq>;
   const f29 X2 = ext_to29(q.x);
   const f29 Y2p = ext_to29(q.y);
   // 64 m - y: lower limbs in (2^30 - 2^29, 2^30 + 2^29), a valid wide operand of mul29
   const f29 Y2 = negate ? neg29(Y2p, P::c64) : Y2p;
   if (empty) {
-    acc = {reduce_value29<Fq>(X2), reduce_value29<Fq>(norm29(Y2)), one29<Fq>(), one29<Fq>()};
+    acc = xyzz29_point(X2, Y2);
     empty = false;
     return;
   }
@@ -115,14 +126,8 @@ BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool 
   mul29x2<Fq>(X2, acc.zz, acc.zzz, Y2, U2, S2);   // X2 ZZ1, ZZZ1 Y2 (Y2 the wide operand)
   const f29 Pd = norm29(sub29(U2, acc.x, P::c2)); // U2 + 2m - X1
   const f29 R = norm29(sub29(S2, acc.y, P::c16)); // S2 + 16m - Y1
-  if (is_zero29<Fq>(Pd)) {
-    if (is_zero29<Fq>(R)) {
-      xyzz29_dbl(acc); // acc equals the point
-    } else {
-      empty = true; // acc = -point
-    }
-    return;
-  }
+  const bool same_x = is_zero29<Fq>(Pd);
+  const bool same_point = same_x && is_zero29<Fq>(R);
   f29 PP, RR, PPP, Q, ZZ3, ZZZ3;
   sqr29x2<Fq>(Pd, R, PP, RR);
   mul29x2<Fq>(Pd, PP, acc.x, PP, PPP, Q);
@@ -133,6 +138,16 @@ BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool 
   const f29 Y3 = mul2_29<Fq>(R, QX, acc.y, NP); // R (Q - X3) - Y1 PPP
   mul29x2<Fq>(acc.zz, PP, acc.zzz, PPP, ZZ3, ZZZ3);
   acc = {X3, Y3, ZZ3, ZZZ3};
+  if (same_x) {
+    if (same_point) { // acc was the point: twice the point
+      const g1_affine p = *again;
+      const f29 y = ext_to29(p.y);
+      acc = xyzz29_point(ext_to29(p.x), negate ? neg29(y, P::c64) : y);
+      xyzz29_dbl(acc);
+    } else { // acc was the negated point
+      empty = true;
+    }
+  }
 }
 
 // Partial sums kept on 29-bit limbs in memory: 4 x 9 words (144 bytes, the

@@ -332,7 +332,7 @@ __global__ __launch_bounds__(ACC_BLOCK) void accumulate_kernel(const job_table j
         cur = b;
         cur_key = key;
       }
-      xyzz29_add_affine(acc, empty, pt0, ((uint32_t)(ent0 >> 32) & SIGN) != 0);
+      xyzz29_add_affine(acc, empty, pt0, ((uint32_t)(ent0 >> 32) & SIGN) != 0, bases + ((uint32_t)(ent0 >> 32) & ~SIGN));
       last_e = e;
     }
     ent0 = ent1;

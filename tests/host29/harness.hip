@@ -143,7 +143,7 @@ void h29_madd_chain(const uint32_t *points, const int *negate, int count, uint32
       q.x.v[i] = points[16 * k + i];
       q.y.v[i] = points[16 * k + 8 + i];
     }
-    xyzz29_add_affine(acc, empty, q, negate[k] != 0);
+    xyzz29_add_affine(acc, empty, q, negate[k] != 0, &q);
     if (!empty) {
       ok = ok && normalized(acc.x) && normalized(acc.y) && normalized(acc.zz) && normalized(acc.zzz);
       ok = ok && below(acc.x, two_m) && below(acc.y, big) && below(acc.zz, big) && below(acc.zzz, big);
