@@ -76,6 +76,41 @@ def test_msm_edge_scalars(bellman, oracle):
     dev.free(db)
 
 
+def test_msm_cancelling_and_repeated_points_in_one_bucket(bellman, oracle):
+    """Equal bases whose scalars are k and r - k meet in the same bucket with
+    opposite signs (the running sum becomes infinity and starts again), equal
+    bases with equal scalars double, and a bucket meets a point, its negation
+    and the point again; the per-bit results equal the oracle's and the fold
+    its closed form."""
+    import random
+
+    B, dev = bellman
+    L = 8
+    n = 1 << L
+    R = oracle.R
+    rng = random.Random(0xBC0CA9CE)
+    s = [rng.randrange(1, R) for _ in range(n)]
+    ks = [rng.randrange(R) for _ in range(n)]
+    k0 = rng.randrange(1, R)
+    s[1] = s[2] = s[0]
+    ks[0], ks[1], ks[2] = k0, R - k0, k0  # +P, -P, +P in each window's bucket
+    s[4] = s[3]
+    ks[4] = R - ks[3]  # cancelling pair
+    s[6] = s[5]
+    ks[6] = ks[5]  # doubling
+    for j in range(16):  # more cancelling pairs, in other buckets and chunks
+        a = 7 + 2 * j
+        s[a + 1] = s[a]
+        ks[a + 1] = (R - ks[a]) % R
+    bases = oracle.gen_bases(s)
+    k = oracle.to_mont(ks)
+    db = dev.upload(bases)
+    got = msm(dev, db, k, L)
+    dev.free(db)
+    assert np.array_equal(oracle.to_affine(got), oracle.to_affine(oracle.msm(bases, k, L)))
+    assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, s, k))
+
+
 def test_msm_config_2_20(bellman, oracle):
     """The 2^20 MSM workload: n = 2^20, scalars 70% uniform / 20% < 2^64 / 10% in {0,1}."""
     B, dev = bellman
