@@ -5,7 +5,7 @@ Jacobian limbs are not canonical), and their fold against the closed form
 import numpy as np
 import pytest
 
-from gpu_helpers import CopyHooks, msm
+from gpu_helpers import CopyHooks, msm, msm_batch
 
 pytestmark = pytest.mark.gpu
 
@@ -106,9 +106,16 @@ def test_msm_cancelling_and_repeated_points_in_one_bucket(bellman, oracle):
     k = oracle.to_mont(ks)
     db = dev.upload(bases)
     got = msm(dev, db, k, L)
-    dev.free(db)
     assert np.array_equal(oracle.to_affine(got), oracle.to_affine(oracle.msm(bases, k, L)))
     assert np.array_equal(oracle.to_affine(oracle.msm_fold(got))[0], closed_form(oracle, s, k))
+    # a batch job with long chunks (32 entries, window 6: a bucket's entries
+    # mostly in one chunk, so one thread meets +P, -P, +P in a row)
+    ds, dr = dev.upload(k), dev.malloc(254 * 96)
+    msm_batch(dev, [B.msm_job(db, ds, dr, L, 6, 32)])
+    got = dev.download(dr, (254, 24))
+    assert np.array_equal(oracle.to_affine(got), oracle.to_affine(oracle.msm(bases, k, L, 6)))
+    for p in (ds, dr, db):
+        dev.free(p)
 
 
 def test_msm_config_2_20(bellman, oracle):
