@@ -135,8 +135,8 @@ BC_HD void xyzz29_add_affine(xyzz29 &acc, bool &empty, const g1_affine &q, bool 
   const f29 X3 = reduce_value29<Fq>(norm29(sub29(RR, W, P::c48e2)));
   const f29 QX = norm29(sub29(Q, X3, P::c2));
   const f29 NP = norm29(neg29(PPP, P::c16));
-  const f29 Y3 = mul2_29<Fq>(R, QX, acc.y, NP); // R (Q - X3) - Y1 PPP
-  mul29x2<Fq>(acc.zz, PP, acc.zzz, PPP, ZZ3, ZZZ3);
+  f29 Y3; // R (Q - X3) - Y1 PPP
+  mul2_mul_mul29<Fq>(R, QX, acc.y, NP, acc.zz, PP, acc.zzz, PPP, Y3, ZZ3, ZZZ3);
   acc = {X3, Y3, ZZ3, ZZZ3};
   if (same_x) {
     if (same_point) { // acc was the point: twice the point

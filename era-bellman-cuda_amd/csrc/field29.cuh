@@ -406,6 +406,21 @@ template <class F> BC_HD void mul29x2(const f29 &a, const f29 &b, const f29 &c, 
 #endif
 }
 
+// r = (a b + c e) / 2^261 (one reduction, all four normalized), s = f g and
+// t = h k (f and k may be the wide operands), the three column chains
+// interleaved on the device.  r, s and t must not alias any input.
+template <class F>
+BC_HD void mul2_mul_mul29(const f29 &a, const f29 &b, const f29 &c, const f29 &e, const f29 &f, const f29 &g, const f29 &h, const f29 &k, f29 &r, f29 &s,
+                          f29 &t) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  mul2_mul_mul29_device<F>(a, b, c, e, f, g, h, k, r, s, t);
+#else
+  r = mul2_29<F>(a, b, c, e);
+  s = mul29<F>(f, g);
+  t = mul29<F>(h, k);
+#endif
+}
+
 // r = a^2 and s = c^2, interleaved on the device as mul29x2
 template <class F> BC_HD void sqr29x2(const f29 &a, const f29 &c, f29 &r, f29 &s) {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -140,6 +140,11 @@ def emit():
                          [("mul", [("a", "b")]), ("mul", [("c", "e")])], ["r", "s"])
     out.append("")
     out += emit_function("sqr29x2_device", "const f29 &a, const f29 &c, f29 &r, f29 &s", [("sqr", "a"), ("sqr", "c")], ["r", "s"])
+    out.append("")
+    out += emit_function("mul2_mul_mul29_device",
+                         "const f29 &a, const f29 &b, const f29 &c, const f29 &e, const f29 &f, const f29 &g, const f29 &h, const f29 &k, "
+                         "f29 &r, f29 &s, f29 &t",
+                         [("mul", [("a", "b"), ("c", "e")]), ("mul", [("f", "g")]), ("mul", [("h", "k")])], ["r", "s", "t"])
     return "\n".join(out) + "\n"
 
 

@@ -54,7 +54,7 @@ def emulate(steps, m, operands):
     N = [(m >> (29 * i)) & M29 if i < 8 else m >> 232 for i in range(9)]
     env = {k: list(v) for k, v in operands.items()}
     env["N"] = N
-    for name in ("q0", "q1", "t", "r", "s"):
+    for name in ("q0", "q1", "q2", "t", "r", "s"):
         env[name] = [0] * 9
     for x in ("a", "c"):
         if x in env:
@@ -128,27 +128,33 @@ def check(name, steps, m, ops):
     elif name == "mul29x2_device":
         montgomery_ok(env["r"], vals["a"] * vals["b"], m)
         montgomery_ok(env["s"], vals["c"] * vals["e"], m)
+    elif name == "mul2_mul_mul29_device":
+        montgomery_ok(env["r"], vals["a"] * vals["b"] + vals["c"] * vals["e"], m)
+        montgomery_ok(env["s"], vals["f"] * vals["g"], m)
+        montgomery_ok(env["t"], vals["h"] * vals["k"], m)
     else:
         assert name == "sqr29x2_device"
         montgomery_ok(env["r"], vals["a"] * vals["a"], m)
         montgomery_ok(env["s"], vals["c"] * vals["c"], m)
 
 
-OPERANDS = {"mul29_device": "ab", "mul2_29_device": "abce", "sqr29_device": "a", "mul29x2_device": "abce", "sqr29x2_device": "ac"}
+OPERANDS = {"mul29_device": "ab", "mul2_29_device": "abce", "sqr29_device": "a", "mul29x2_device": "abce", "sqr29x2_device": "ac",
+            "mul2_mul_mul29_device": "abcefghk"}
 
 
 def test_emulated_products_are_montgomery_products():
     fns = functions()
     assert set(fns) == set(OPERANDS)
     mads = {n: sum(len([c for c in s[1] if c.startswith("v_mad")]) for s in st if s[0] == "asm") for n, st in fns.items()}
-    assert mads == {"mul29_device": 162, "mul2_29_device": 243, "sqr29_device": 126, "mul29x2_device": 324, "sqr29x2_device": 252}
+    assert mads == {"mul29_device": 162, "mul2_29_device": 243, "sqr29_device": 126, "mul29x2_device": 324, "sqr29x2_device": 252,
+                    "mul2_mul_mul29_device": 567}
     rng = random.Random(0xBC29)
     for m in moduli().values():
         top = (1 << 260) - 1  # inputs below 2^260: top limbs below 2^28
-        cases = [[top] * 4, [0] * 4, [m - 1] * 4, [1, m - 1, 2 * m - 1, 3 * m]]
-        cases += [[rng.randrange(1 << 260) for _ in range(4)] for _ in range(200)]
+        cases = [[top] * 8, [0] * 8, [m - 1] * 8, [1, m - 1, 2 * m - 1, 3 * m] * 2]
+        cases += [[rng.randrange(1 << 260) for _ in range(8)] for _ in range(200)]
         for xs in cases:
-            vals = dict(zip("abce", (limbs(x) for x in xs)))
+            vals = dict(zip("abcefghk", (limbs(x) for x in xs)))
             for name, steps in fns.items():
                 check(name, steps, m, {k: vals[k] for k in OPERANDS[name]})
         # every limb at its bound (normalized lower limbs at 2^29 - 1), and
@@ -163,6 +169,8 @@ def test_emulated_products_are_montgomery_products():
         check("mul2_29_device", fns["mul2_29_device"], m, {"a": full, "b": full, "c": full, "e": full})
         check("sqr29_device", fns["sqr29_device"], m, {"a": full})
         check("sqr29x2_device", fns["sqr29x2_device"], m, {"a": full, "c": full})
+        check("mul2_mul_mul29_device", fns["mul2_mul_mul29_device"], m,
+              {"a": full, "b": full, "c": full, "e": full, "f": wide, "g": full, "h": full, "k": wide})
 
 
 def test_emulation_detects_a_wrong_statement():
